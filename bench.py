@@ -1,0 +1,209 @@
+"""Benchmark of the hip family on one MI355X.
+
+A step is one pass of the hot path over the batched workload: 200,000
+independent protein pairs (lengths log-normal, median 300, sigma 0.6,
+clipped to [20, 5000], Robinson-Robinson residues, BLOSUM62, open 11
+extend 1), aligned score-only with SW, NW and SG (all four ends free), each
+in the int16-first tiering of parasail_*_striped_profile_sat, inputs resident
+in HBM.  Prints one JSON line (GCUPS over the three algorithms), with the
+plain one-launch-per-call kernel (1 query of 300 vs 2,000 targets of
+U[50,1000], parasail_sw_striped_profile_16 semantics) beside it, the
+packed-int16 VALU roofline of the dominant kernel and the CPU oracle timed
+on the host cores over a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs N]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import parasail_amd as P  # noqa: E402
+from parasail_amd import workloads as WL  # noqa: E402
+
+# VALU ops per cell, fixed from the recurrence: SW 10, NW/SG 9
+OPS = {"sw": 10, "nw": 9, "sg": 9}
+ALGS = ("sw", "nw", "sg")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_resident(batch, matrix):
+    L = P.lib()
+    L.parasail_hip_batch_create.restype = ctypes.c_void_p
+    L.parasail_hip_batch_create.argtypes = [
+        ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int),
+        ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int),
+        ctypes.c_int, P.MP]
+    L.parasail_hip_batch_align.restype = ctypes.c_int
+    L.parasail_hip_batch_align.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
+                                           ctypes.c_int, ctypes.POINTER(P.BatchResult)]
+    L.parasail_hip_batch_free.argtypes = [ctypes.c_void_p]
+    I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
+    h = L.parasail_hip_batch_create(batch.q, batch.qoff.ctypes.data_as(I64),
+                                    batch.qlen.ctypes.data_as(I32), batch.t,
+                                    batch.toff.ctypes.data_as(I64),
+                                    batch.tlen.ctypes.data_as(I32), len(batch), matrix)
+    if not h:
+        raise RuntimeError("parasail_hip_batch_create failed: %s"
+                           % L.parasail_hip_last_error().decode(errors="replace"))
+    return h
+
+
+def run_resident(h, alg, n, keep=None):
+    L = P.lib()
+    br = P.BatchResult()
+    out = keep if keep is not None else {k: np.zeros(n, dtype=np.int32)
+                                         for k in ("score", "end_query", "end_ref", "flag")}
+    for k, v in out.items():
+        setattr(br, k, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if L.parasail_hip_batch_align(h, alg.encode(), 11, 1, ctypes.byref(br)) != 0:
+        raise RuntimeError("parasail_hip_batch_align(%s) failed" % alg)
+    return out, P.last_kernel_ms()
+
+
+def per_call_gcups(reps=1):
+    """Config 0: parasail_sw_striped_profile_hip_64_16, one launch per call."""
+    L = P.lib()
+    q, b = WL.config0()
+    m = P.blosum62()
+    create = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, P.MP)(
+        L.parasail_lookup_pcreator(b"parasail_sw_striped_profile_hip_64_16"))
+    pf = ctypes.CFUNCTYPE(P.RP, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_int)(
+        L.parasail_lookup_pfunction(b"parasail_sw_striped_profile_hip_64_16"))
+    L.parasail_profile_free.argtypes = [ctypes.c_void_p]
+    prof = create(q, len(q), m)
+    targets = [b.target(k) for k in range(len(b))]
+    for t in targets[:20]:
+        L.parasail_result_free(pf(prof, t, len(t), 11, 1))
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for t in targets:
+            L.parasail_result_free(pf(prof, t, len(t), 11, 1))
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    L.parasail_profile_free(prof)
+    return b.cells / best / 1e9, b.cells, best
+
+
+def cpu_baseline(batch, target_s=10.0):
+    """The CPU oracle (serial recurrence per pair, OpenMP over pairs:
+    tests/test_openmp.c:383-441) on a bounded sample of the batch."""
+    import oracle as O
+    om = O.blosum62()
+    threads = os.cpu_count() or 1
+    rng = np.random.default_rng(9)
+    idx = rng.choice(len(batch), size=min(len(batch), 200), replace=False)
+    algs = {"sw": (O.SW, (0, 0, 0, 0)), "nw": (O.NW, (0, 0, 0, 0)), "sg": (O.SG, (1, 1, 1, 1))}
+
+    def run(ids):
+        qs = [om.map(batch.query(k)) for k in ids]
+        ts = [om.map(batch.target(k)) for k in ids]
+        cells = sum(len(a) * len(b) for a, b in zip(qs, ts))
+        secs = 0.0
+        for alg, flags in algs.values():
+            _, s = O.batch(alg, O.SCORE, qs, ts, 11, 1, om, flags, threads=threads)
+            secs += s
+        return 3 * cells, secs
+
+    cells, secs = run(idx)
+    scale = max(1, int(target_s / max(secs, 1e-3)))
+    n2 = min(len(batch), len(idx) * scale)
+    idx = rng.choice(len(batch), size=n2, replace=False)
+    cells, secs = run(idx)
+    return {"value": round(cells / secs / 1e9, 3), "unit": "GCUPS", "cores": threads,
+            "kind": "port",
+            "sample": "%d random pairs of the batch x (SW, NW, SG), serial int32 oracle, "
+                      "OpenMP over pairs, %.1f s" % (n2, secs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=200000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-percall", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every configuration of this path fits one MI355X and "
+              "no multi-GPU figure is produced (pairs are independent; see the sharding "
+              "design).", file=sys.stderr)
+        sys.exit(2)
+    if not P.available():
+        raise RuntimeError("no HIP device: %s" % P.lib().parasail_hip_last_error())
+    cus, clk = ctypes.c_int(), ctypes.c_int()
+    P.lib().parasail_hip_device_info(ctypes.byref(cus), ctypes.byref(clk))
+    peak_ops = cus.value * clk.value * 1e3 * 128  # packed int16: 64 lanes x 2 halves
+    log("device: %d CUs at %.0f MHz; packed-int16 VALU peak %.2f TOP/s"
+        % (cus.value, clk.value / 1e3, peak_ops / 1e12))
+
+    t0 = time.perf_counter()
+    batch = WL.config1(npairs=a.pairs)
+    cells = batch.cells
+    log("workload: %d pairs, %.3e cells per algorithm (generated in %.1f s)"
+        % (len(batch), cells, time.perf_counter() - t0))
+    m = P.blosum62()
+    h = make_resident(batch, m)
+    bufs = {alg: {k: np.zeros(len(batch), dtype=np.int32)
+                  for k in ("score", "end_query", "end_ref", "flag")} for alg in ALGS}
+    for w in range(a.warmup):
+        for alg in ALGS:
+            run_resident(h, alg, len(batch), bufs[alg])
+        log("warmup %d done" % w)
+    kms = {alg: [] for alg in ALGS}
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        for alg in ALGS:
+            _, ms = run_resident(h, alg, len(batch), bufs[alg])
+            kms[alg].append(ms)
+        log("step %d done" % s)
+    elapsed = time.perf_counter() - t0
+    P.lib().parasail_hip_batch_free(h)
+    ms_per_step = elapsed / a.steps * 1e3
+    value = 3 * cells / (elapsed / a.steps) / 1e9
+    per_alg = {alg: round(cells / (np.mean(kms[alg]) / 1e3) / 1e9, 1) for alg in ALGS}
+    sw_s = np.mean(kms["sw"]) / 1e3
+    achieved = cells * OPS["sw"] / sw_s
+    w32 = {alg: int(np.count_nonzero(bufs[alg]["flag"] & P.FLAG_BITS_32)) for alg in ALGS}
+    line = {
+        "metric": "GCUPS batched vs per-pair kernel on 1 MI355X, bit-exact; serving p50/p99 ms",
+        "value": round(value, 1), "unit": "GCUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_per_step, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+        "config": {"workload": "200k protein pairs, log-normal(300, 0.6) lengths, "
+                               "BLOSUM62 11/1, SW+NW+SG score-only, int16-first sat",
+                   "pairs": len(batch), "cells_per_alg": cells},
+        "gcups_kernel_by_alg": per_alg,
+        "pairs_int32_tier": w32,
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 2),
+                     "peak": round(peak_ops / 1e12, 2), "unit": "TOP/s",
+                     "frac": round(achieved / peak_ops, 4), "traffic": None,
+                     "kernel": "k_wave16 (SW, all length bins)",
+                     "ops_per_cell": OPS["sw"]},
+    }
+    if not a.no_percall:
+        g, c0, s0 = per_call_gcups()
+        line["per_pair_kernel"] = {"gcups": round(g, 3), "cells": c0, "seconds": round(s0, 3),
+                                   "function": "parasail_sw_striped_profile_hip_64_16",
+                                   "workload": "1 x 300 aa query vs 2,000 targets U[50,1000]"}
+        line["batched_over_per_pair_sw"] = round(per_alg["sw"] / g, 1)
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(batch)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

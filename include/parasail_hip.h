@@ -1,0 +1,411 @@
+/*
+ * parasail_hip.h -- the "hip" ISA family of parasail for AMD Instinct
+ * MI355X (gfx950), as a C ABI.
+ *
+ * Drop-in boundary.  Every entry point below has the exact signature of the
+ * reference interface it replaces:
+ *   parasail_function_t   parasail.h:184-188  (s1, s1Len, s2, s2Len, open, gap, matrix)
+ *   parasail_pfunction_t  parasail.h:206-209  (profile, s2, s2Len, open, gap)
+ *   parasail_pcreator_t   parasail.h:211-213  (s1, s1Len, matrix)
+ *   parasail_nw_banded    parasail.h:293-297
+ * and returns the reference's own object types (parasail_result_t,
+ * parasail_cigar_t, parasail_profile_t, parasail_matrix_t; parasail.h:85-230)
+ * allocated with the reference's allocators, so parasail_result_free() of
+ * either library releases them.  Names follow the reference grammar
+ * alg[_stats|_trace]_{striped|scan}[_profile]_{isa}_{bits}_{width}
+ * (README.md:246-281) with isa "hip" and bits "64" (one wave64).
+ *
+ * The library also exports the data-model half of parasail's API (result
+ * getters, matrices, profiles, CIGAR, function lookup) with the reference's
+ * names and semantics, so a program that uses only the alignment path links
+ * against this library alone.
+ */
+#ifndef PARASAIL_HIP_H
+#define PARASAIL_HIP_H
+
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef _PARASAIL_H_
+/* ---- types and flags, ABI-identical to parasail.h:31-230 ---- */
+#define PARASAIL_VERSION_MAJOR 2
+#define PARASAIL_VERSION_MINOR 6
+#define PARASAIL_VERSION_PATCH 2
+
+#define PARASAIL_ZERO_MASK 120
+#define PARASAIL_E_MASK 103
+#define PARASAIL_F_MASK 31
+#define PARASAIL_ZERO 0
+#define PARASAIL_INS 1
+#define PARASAIL_DEL 2
+#define PARASAIL_DIAG 4
+#define PARASAIL_DIAG_E 8
+#define PARASAIL_INS_E 16
+#define PARASAIL_DIAG_F 32
+#define PARASAIL_DEL_F 64
+
+#define PARASAIL_FLAG_NW (1 << 0)
+#define PARASAIL_FLAG_SG (1 << 1)
+#define PARASAIL_FLAG_SW (1 << 2)
+#define PARASAIL_FLAG_SG_S1_BEG (1 << 3)
+#define PARASAIL_FLAG_SG_S1_END (1 << 4)
+#define PARASAIL_FLAG_SATURATED (1 << 6)
+#define PARASAIL_FLAG_BANDED (1 << 7)
+#define PARASAIL_FLAG_NOVEC (1 << 8)
+#define PARASAIL_FLAG_NOVEC_SCAN (1 << 9)
+#define PARASAIL_FLAG_SCAN (1 << 10)
+#define PARASAIL_FLAG_STRIPED (1 << 11)
+#define PARASAIL_FLAG_DIAG (1 << 12)
+#define PARASAIL_FLAG_BLOCKED (1 << 13)
+#define PARASAIL_FLAG_SG_S2_BEG (1 << 14)
+#define PARASAIL_FLAG_SG_S2_END (1 << 15)
+#define PARASAIL_FLAG_STATS (1 << 16)
+#define PARASAIL_FLAG_TABLE (1 << 17)
+#define PARASAIL_FLAG_ROWCOL (1 << 18)
+#define PARASAIL_FLAG_TRACE (1 << 19)
+#define PARASAIL_FLAG_BITS_8 (1 << 20)
+#define PARASAIL_FLAG_BITS_16 (1 << 21)
+#define PARASAIL_FLAG_BITS_32 (1 << 22)
+#define PARASAIL_FLAG_BITS_64 (1 << 23)
+#define PARASAIL_FLAG_LANES_1 (1 << 24)
+#define PARASAIL_FLAG_LANES_2 (1 << 25)
+#define PARASAIL_FLAG_LANES_4 (1 << 26)
+#define PARASAIL_FLAG_LANES_8 (1 << 27)
+#define PARASAIL_FLAG_LANES_16 (1 << 28)
+#define PARASAIL_FLAG_LANES_32 (1 << 29)
+#define PARASAIL_FLAG_LANES_64 (1 << 30)
+#define PARASAIL_FLAG_INVALID 0x80000020
+
+#define PARASAIL_MATRIX_TYPE_SQUARE 0
+#define PARASAIL_MATRIX_TYPE_PSSM 1
+
+typedef struct parasail_result_extra_stats_tables {
+    int *score_table, *matches_table, *similar_table, *length_table;
+} parasail_result_extra_stats_tables_t;
+
+typedef struct parasail_result_extra_stats_rowcols {
+    int *score_row, *matches_row, *similar_row, *length_row;
+    int *score_col, *matches_col, *similar_col, *length_col;
+} parasail_result_extra_stats_rowcols_t;
+
+typedef struct parasail_result_extra_stats {
+    int matches;
+    int similar;
+    int length;
+    union {
+        void *extra;
+        parasail_result_extra_stats_tables_t *tables;
+        parasail_result_extra_stats_rowcols_t *rowcols;
+    };
+} parasail_result_extra_stats_t;
+
+typedef struct parasail_result_extra_tables {
+    int *score_table;
+} parasail_result_extra_tables_t;
+
+typedef struct parasail_result_extra_rowcols {
+    int *score_row;
+    int *score_col;
+} parasail_result_extra_rowcols_t;
+
+typedef struct parasail_result_extra_trace {
+    void *trace_table;
+    void *trace_ins_table;
+    void *trace_del_table;
+} parasail_result_extra_trace_t;
+
+typedef struct parasail_result {
+    int score;
+    int end_query;
+    int end_ref;
+    int flag;
+    union {
+        void *extra;
+        parasail_result_extra_stats_t *stats;
+        parasail_result_extra_tables_t *tables;
+        parasail_result_extra_rowcols_t *rowcols;
+        parasail_result_extra_trace_t *trace;
+    };
+} parasail_result_t;
+
+typedef struct parasail_matrix {
+    const char *name;
+    const int *matrix;
+    const int *mapper;
+    int size;
+    int max;
+    int min;
+    int *user_matrix;
+    int type;
+    int length;
+    const char *alphabet;
+    const char *query;
+} parasail_matrix_t;
+
+typedef struct parasail_profile_data {
+    void *score;
+    void *matches;
+    void *similar;
+} parasail_profile_data_t;
+
+typedef struct parasail_profile {
+    const char *s1;
+    int s1Len;
+    const parasail_matrix_t *matrix;
+    struct parasail_profile_data profile8;
+    struct parasail_profile_data profile16;
+    struct parasail_profile_data profile32;
+    struct parasail_profile_data profile64;
+    void (*free)(void *profile);
+    int stop;
+} parasail_profile_t;
+
+typedef parasail_result_t *parasail_function_t(const char *const s1, const int s1Len,
+                                               const char *const s2, const int s2Len,
+                                               const int open, const int gap,
+                                               const parasail_matrix_t *matrix);
+
+typedef struct parasail_function_info {
+    parasail_function_t *pointer;
+    const char *name;
+    const char *alg;
+    const char *type;
+    const char *isa;
+    const char *bits;
+    const char *width;
+    int lanes;
+    char is_table;
+    char is_rowcol;
+    char is_trace;
+    char is_stats;
+    char is_ref;
+} parasail_function_info_t;
+
+typedef parasail_result_t *parasail_pfunction_t(const parasail_profile_t *const profile,
+                                                const char *const s2, const int s2Len,
+                                                const int open, const int gap);
+
+typedef parasail_profile_t *parasail_pcreator_t(const char *const s1, const int s1Len,
+                                                const parasail_matrix_t *matrix);
+
+typedef struct parasail_pfunction_info {
+    parasail_pfunction_t *pointer;
+    parasail_pcreator_t *creator;
+    const char *name;
+    const char *alg;
+    const char *type;
+    const char *isa;
+    const char *bits;
+    const char *width;
+    int lanes;
+    char is_table;
+    char is_rowcol;
+    char is_trace;
+    char is_stats;
+    char is_ref;
+} parasail_pfunction_info_t;
+
+typedef struct parasail_cigar_ {
+    uint32_t *seq;
+    int len;
+    int beg_query;
+    int beg_ref;
+} parasail_cigar_t;
+
+/* ---- data model (reference names; parasail.h:232-518, src/memory.c) ---- */
+void parasail_version(int *major, int *minor, int *patch);
+double parasail_time(void);
+void parasail_result_free(parasail_result_t *result);
+void parasail_profile_free(parasail_profile_t *profile);
+parasail_function_t *parasail_lookup_function(const char *funcname);
+parasail_pfunction_t *parasail_lookup_pfunction(const char *funcname);
+parasail_pcreator_t *parasail_lookup_pcreator(const char *funcname);
+const parasail_function_info_t *parasail_lookup_function_info(const char *funcname);
+const parasail_pfunction_info_t *parasail_lookup_pfunction_info(const char *funcname);
+const parasail_matrix_t *parasail_matrix_lookup(const char *matrixname);
+parasail_matrix_t *parasail_matrix_create(const char *alphabet, const int match, const int mismatch);
+parasail_matrix_t *parasail_matrix_create_case_sensitive(const char *alphabet, const int match,
+                                                         const int mismatch);
+parasail_matrix_t *parasail_matrix_pssm_create(const char *alphabet, const int *values,
+                                               const int length);
+parasail_matrix_t *parasail_matrix_pssm_create_case_sensitive(const char *alphabet,
+                                                              const int *values, const int length);
+void parasail_matrix_free(parasail_matrix_t *matrix);
+parasail_matrix_t *parasail_matrix_copy(const parasail_matrix_t *original);
+parasail_matrix_t *parasail_matrix_convert_square_to_pssm(const parasail_matrix_t *original,
+                                                          const char *seq, int len);
+void parasail_matrix_set_value(parasail_matrix_t *matrix, int row, int col, int value);
+
+uint32_t parasail_cigar_encode(uint32_t length, char op_letter);
+parasail_cigar_t *parasail_cigar_encode_string(const char *cigar);
+char parasail_cigar_decode_op(uint32_t cigar_int);
+uint32_t parasail_cigar_decode_len(uint32_t cigar_int);
+char *parasail_cigar_decode(parasail_cigar_t *cigar);
+parasail_cigar_t *parasail_result_get_cigar(parasail_result_t *result, const char *seqA, int lena,
+                                            const char *seqB, int lenb,
+                                            const parasail_matrix_t *matrix);
+parasail_cigar_t *parasail_result_get_cigar_extra(parasail_result_t *result, const char *seqA,
+                                                  int lena, const char *seqB, int lenb,
+                                                  const parasail_matrix_t *matrix,
+                                                  int case_sensitive,
+                                                  const char *alphabet_aliases);
+void parasail_cigar_free(parasail_cigar_t *cigar);
+
+int parasail_result_is_nw(const parasail_result_t *const result);
+int parasail_result_is_sg(const parasail_result_t *const result);
+int parasail_result_is_sw(const parasail_result_t *const result);
+int parasail_result_is_saturated(const parasail_result_t *const result);
+int parasail_result_is_banded(const parasail_result_t *const result);
+int parasail_result_is_scan(const parasail_result_t *const result);
+int parasail_result_is_striped(const parasail_result_t *const result);
+int parasail_result_is_diag(const parasail_result_t *const result);
+int parasail_result_is_blocked(const parasail_result_t *const result);
+int parasail_result_is_stats(const parasail_result_t *const result);
+int parasail_result_is_stats_table(const parasail_result_t *const result);
+int parasail_result_is_stats_rowcol(const parasail_result_t *const result);
+int parasail_result_is_table(const parasail_result_t *const result);
+int parasail_result_is_rowcol(const parasail_result_t *const result);
+int parasail_result_is_trace(const parasail_result_t *const result);
+int parasail_result_get_score(const parasail_result_t *const result);
+int parasail_result_get_end_query(const parasail_result_t *const result);
+int parasail_result_get_end_ref(const parasail_result_t *const result);
+int parasail_result_get_matches(const parasail_result_t *const result);
+int parasail_result_get_similar(const parasail_result_t *const result);
+int parasail_result_get_length(const parasail_result_t *const result);
+int *parasail_result_get_trace_table(const parasail_result_t *const result);
+
+/* striped trace table -> row-major (src/striped_unwind.c:5-26,
+ * parasail/memory.h:137-141) */
+int *parasail_striped_unwind(int lena, int lenb, parasail_result_t *result);
+#endif /* _PARASAIL_H_ */
+
+/* ---- the hip family (drop-ins) ----
+ *
+ * For every ALG in {sw, nw, sg, sg_qb, sg_qe, sg_qx, sg_db, sg_de, sg_dx,
+ * sg_qb_de, sg_qe_db, sg_qb_db, sg_qe_de} (src/sg_helper.h:41-87 maps the
+ * sg_* names to the four end-gap flags), KIND in {"", _stats, _trace},
+ * STRATEGY in {striped, scan} and WIDTH in {8, 16, 32, sat}:
+ *
+ *   parasail_function_t  parasail_<ALG><KIND>_<STRATEGY>_hip_64_<WIDTH>
+ *       replaces parasail_<ALG><KIND>_<STRATEGY>_<isa>_<bits>_<WIDTH>,
+ *       e.g. src/sw_striped_avx2_256_16.c:140-422
+ *   parasail_pfunction_t parasail_<ALG><KIND>_<STRATEGY>_profile_hip_64_<WIDTH>
+ *       replaces ..._profile_avx2_256_<WIDTH> (parasail.h:206-209)
+ *
+ * Widths: every result is computed exactly; a width's window decides only
+ * whether the result is reported as saturated (PARASAIL_FLAG_SATURATED with
+ * the reference's saturated outputs) and which BITS_* flag it carries.
+ * "sat" reports the narrowest width whose window holds the alignment
+ * (src/satcheck.c:41520-41537).  Unavailable device: NULL and errno=ENOSYS
+ * (util/isastubs.py:23-31).
+ */
+#define PARASAIL_HIP_ALGS(X)                                                  \
+    X(sw) X(nw) X(sg) X(sg_qb) X(sg_qe) X(sg_qx) X(sg_db) X(sg_de) X(sg_dx)     \
+    X(sg_qb_de) X(sg_qe_db) X(sg_qb_db) X(sg_qe_de)
+
+#define PARASAIL_HIP_DECL_W(alg, kind, strat, w)                              \
+    parasail_result_t *parasail_##alg##kind##_##strat##_hip_64_##w(             \
+        const char *const s1, const int s1Len, const char *const s2,           \
+        const int s2Len, const int open, const int gap,                        \
+        const parasail_matrix_t *matrix);                                      \
+    parasail_result_t *parasail_##alg##kind##_##strat##_profile_hip_64_##w(     \
+        const parasail_profile_t *const profile, const char *const s2,         \
+        const int s2Len, const int open, const int gap);
+
+#define PARASAIL_HIP_DECL_S(alg, kind, strat)                                 \
+    PARASAIL_HIP_DECL_W(alg, kind, strat, 8)                                  \
+    PARASAIL_HIP_DECL_W(alg, kind, strat, 16)                                 \
+    PARASAIL_HIP_DECL_W(alg, kind, strat, 32)                                 \
+    PARASAIL_HIP_DECL_W(alg, kind, strat, sat)
+
+#define PARASAIL_HIP_DECL_ALG(alg)                                            \
+    PARASAIL_HIP_DECL_S(alg, , striped) PARASAIL_HIP_DECL_S(alg, , scan)       \
+    PARASAIL_HIP_DECL_S(alg, _stats, striped)                                 \
+    PARASAIL_HIP_DECL_S(alg, _stats, scan)                                    \
+    PARASAIL_HIP_DECL_S(alg, _trace, striped)                                 \
+    PARASAIL_HIP_DECL_S(alg, _trace, scan)
+
+PARASAIL_HIP_ALGS(PARASAIL_HIP_DECL_ALG)
+
+/* profile creators, replacing parasail_profile_create[_stats]_avx_256_<W>
+ * (src/memory_avx2.c:81-132, :240-253, :255-318).  The profile keeps the
+ * mapped query and substitution table resident in device memory. */
+parasail_profile_t *parasail_profile_create_hip_64_8(const char *const s1, const int s1Len,
+                                                     const parasail_matrix_t *matrix);
+parasail_profile_t *parasail_profile_create_hip_64_16(const char *const s1, const int s1Len,
+                                                      const parasail_matrix_t *matrix);
+parasail_profile_t *parasail_profile_create_hip_64_32(const char *const s1, const int s1Len,
+                                                      const parasail_matrix_t *matrix);
+parasail_profile_t *parasail_profile_create_hip_64_sat(const char *const s1, const int s1Len,
+                                                       const parasail_matrix_t *matrix);
+parasail_profile_t *parasail_profile_create_stats_hip_64_8(const char *const s1, const int s1Len,
+                                                           const parasail_matrix_t *matrix);
+parasail_profile_t *parasail_profile_create_stats_hip_64_16(const char *const s1, const int s1Len,
+                                                            const parasail_matrix_t *matrix);
+parasail_profile_t *parasail_profile_create_stats_hip_64_32(const char *const s1, const int s1Len,
+                                                            const parasail_matrix_t *matrix);
+parasail_profile_t *parasail_profile_create_stats_hip_64_sat(const char *const s1, const int s1Len,
+                                                             const parasail_matrix_t *matrix);
+
+/* banded global alignment, replacing parasail_nw_banded (parasail.h:293-297,
+ * src/nw_banded.c:29-189) */
+parasail_result_t *parasail_nw_banded_hip(const char *const s1, const int s1Len,
+                                          const char *const s2, const int s2Len,
+                                          const int open, const int gap, const int k,
+                                          const parasail_matrix_t *matrix);
+
+/* ---- batched entry points (no reference counterpart; the reference's
+ * callers loop over pairs with OpenMP, tests/test_openmp.c:383-441) ----
+ *
+ * Aligns npairs (s1[k], s2[k]) in one device pass.  alg_name is any alg
+ * above ("sw", "sg_qx", ...); kind 0 score, 1 stats, 2 trace.  Sequences
+ * are concatenated with per-pair offsets and lengths.  Results go to flat
+ * arrays (any may be NULL): score, end_query, end_ref, matches, similar,
+ * length, and flags (the PARASAIL_FLAG_* a per-pair call would set).  For
+ * kind 2, cigars[k] receives a parasail_cigar_t (free with
+ * parasail_cigar_free) when cigars is non-NULL.  Returns 0, or -1 with a
+ * message in parasail_hip_last_error().
+ */
+typedef struct parasail_hip_batch_result {
+    int *score, *end_query, *end_ref;
+    int *matches, *similar, *length;
+    int *flag;
+    parasail_cigar_t **cigars;
+} parasail_hip_batch_result_t;
+
+int parasail_hip_align_batch(const char *alg_name, int kind, int width, const char *s1,
+                             const int64_t *s1_off, const int *s1_len, const char *s2,
+                             const int64_t *s2_off, const int *s2_len, int npairs, int open,
+                             int gap, const parasail_matrix_t *matrix,
+                             parasail_hip_batch_result_t *out);
+
+/* A batch kept resident in device memory (HBM): the residues are mapped and
+ * uploaded once, and every parasail_hip_batch_align call over it launches
+ * kernels only.  Score-only (kind 0) results, as parasail_hip_align_batch
+ * with width "sat" returns them.  NULL / -1 on failure. */
+typedef struct parasail_hip_batch parasail_hip_batch_t;
+
+parasail_hip_batch_t *parasail_hip_batch_create(const char *s1, const int64_t *s1_off,
+                                                const int *s1_len, const char *s2,
+                                                const int64_t *s2_off, const int *s2_len,
+                                                int npairs, const parasail_matrix_t *matrix);
+int parasail_hip_batch_align(parasail_hip_batch_t *batch, const char *alg_name, int open,
+                             int gap, parasail_hip_batch_result_t *out);
+void parasail_hip_batch_free(parasail_hip_batch_t *batch);
+
+/* device time (ms, HIP events) of the last call's alignment kernels */
+double parasail_hip_last_kernel_ms(void);
+const char *parasail_hip_last_error(void);
+/* 1 if a HIP device is usable, 0 otherwise (never falls back to the CPU) */
+int parasail_hip_available(void);
+/* compute units and peak engine clock (kHz) of the device in use */
+int parasail_hip_device_info(int *compute_units, int *clock_khz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARASAIL_HIP_H */

@@ -1,0 +1,233 @@
+"""Python view of libparasail_hip.so (the hip family of parasail on MI355X).
+
+The library's C ABI is declared in include/parasail_hip.h.  This module only
+loads it through ctypes and wraps a few calls for tests and the benchmark;
+there is no Python or CPU fallback: when the shared library or the device is
+missing, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libparasail_hip.so")
+
+FLAG_NW, FLAG_SG, FLAG_SW = 1 << 0, 1 << 1, 1 << 2
+FLAG_SG_S1_BEG, FLAG_SG_S1_END = 1 << 3, 1 << 4
+FLAG_SATURATED = 1 << 6
+FLAG_BANDED = 1 << 7
+FLAG_SCAN, FLAG_STRIPED = 1 << 10, 1 << 11
+FLAG_SG_S2_BEG, FLAG_SG_S2_END = 1 << 14, 1 << 15
+FLAG_STATS, FLAG_TRACE = 1 << 16, 1 << 19
+FLAG_BITS_8, FLAG_BITS_16, FLAG_BITS_32 = 1 << 20, 1 << 21, 1 << 22
+FLAG_LANES_64 = 1 << 30
+
+ALGS = ["sw", "nw", "sg", "sg_qb", "sg_qe", "sg_qx", "sg_db", "sg_de", "sg_dx",
+        "sg_qb_de", "sg_qe_db", "sg_qb_db", "sg_qe_de"]
+# (s1_beg, s1_end, s2_beg, s2_end) of every sg name (src/sg_helper.h:53-63)
+SG_FLAGS = {"sg": (1, 1, 1, 1), "sg_qb": (1, 0, 0, 0), "sg_qe": (0, 1, 0, 0),
+            "sg_qx": (1, 1, 0, 0), "sg_db": (0, 0, 1, 0), "sg_de": (0, 0, 0, 1),
+            "sg_dx": (0, 0, 1, 1), "sg_qb_de": (1, 0, 0, 1), "sg_qe_db": (0, 1, 1, 0),
+            "sg_qb_db": (1, 0, 1, 0), "sg_qe_de": (0, 1, 0, 1)}
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("score", ctypes.c_int), ("end_query", ctypes.c_int),
+                ("end_ref", ctypes.c_int), ("flag", ctypes.c_int),
+                ("extra", ctypes.c_void_p)]
+
+
+class StatsExtra(ctypes.Structure):
+    _fields_ = [("matches", ctypes.c_int), ("similar", ctypes.c_int),
+                ("length", ctypes.c_int), ("extra", ctypes.c_void_p)]
+
+
+class TraceExtra(ctypes.Structure):
+    _fields_ = [("trace_table", ctypes.c_void_p), ("trace_ins_table", ctypes.c_void_p),
+                ("trace_del_table", ctypes.c_void_p)]
+
+
+class Matrix(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("matrix", ctypes.POINTER(ctypes.c_int)),
+                ("mapper", ctypes.POINTER(ctypes.c_int)), ("size", ctypes.c_int),
+                ("max", ctypes.c_int), ("min", ctypes.c_int),
+                ("user_matrix", ctypes.POINTER(ctypes.c_int)), ("type", ctypes.c_int),
+                ("length", ctypes.c_int), ("alphabet", ctypes.c_char_p),
+                ("query", ctypes.c_char_p)]
+
+
+class Cigar(ctypes.Structure):
+    _fields_ = [("seq", ctypes.POINTER(ctypes.c_uint32)), ("len", ctypes.c_int),
+                ("beg_query", ctypes.c_int), ("beg_ref", ctypes.c_int)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("score", ctypes.POINTER(ctypes.c_int)), ("end_query", ctypes.POINTER(ctypes.c_int)),
+                ("end_ref", ctypes.POINTER(ctypes.c_int)), ("matches", ctypes.POINTER(ctypes.c_int)),
+                ("similar", ctypes.POINTER(ctypes.c_int)), ("length", ctypes.POINTER(ctypes.c_int)),
+                ("flag", ctypes.POINTER(ctypes.c_int)),
+                ("cigars", ctypes.POINTER(ctypes.POINTER(Cigar)))]
+
+
+RP = ctypes.POINTER(Result)
+MP = ctypes.POINTER(Matrix)
+FN = ctypes.CFUNCTYPE(RP, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_int, MP)
+
+_lib = None
+
+
+def lib():
+    """Load libparasail_hip.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libparasail_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.parasail_result_free.argtypes = [RP]
+        L.parasail_lookup_function.restype = ctypes.c_void_p
+        L.parasail_lookup_function.argtypes = [ctypes.c_char_p]
+        L.parasail_lookup_pfunction.restype = ctypes.c_void_p
+        L.parasail_lookup_pfunction.argtypes = [ctypes.c_char_p]
+        L.parasail_lookup_pcreator.restype = ctypes.c_void_p
+        L.parasail_lookup_pcreator.argtypes = [ctypes.c_char_p]
+        L.parasail_matrix_lookup.restype = MP
+        L.parasail_matrix_lookup.argtypes = [ctypes.c_char_p]
+        L.parasail_matrix_create.restype = MP
+        L.parasail_matrix_create.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        L.parasail_matrix_pssm_create.restype = MP
+        L.parasail_matrix_pssm_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                                  ctypes.c_int]
+        L.parasail_matrix_convert_square_to_pssm.restype = MP
+        L.parasail_matrix_convert_square_to_pssm.argtypes = [MP, ctypes.c_char_p, ctypes.c_int]
+        L.parasail_matrix_free.argtypes = [MP]
+        L.parasail_result_get_cigar.restype = ctypes.POINTER(Cigar)
+        L.parasail_result_get_cigar.argtypes = [RP, ctypes.c_char_p, ctypes.c_int,
+                                                ctypes.c_char_p, ctypes.c_int, MP]
+        L.parasail_cigar_decode.restype = ctypes.c_void_p
+        L.parasail_cigar_decode.argtypes = [ctypes.POINTER(Cigar)]
+        L.parasail_cigar_free.argtypes = [ctypes.POINTER(Cigar)]
+        L.parasail_nw_banded_hip.restype = RP
+        L.parasail_nw_banded_hip.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, MP]
+        L.parasail_hip_align_batch.restype = ctypes.c_int
+        L.parasail_hip_align_batch.argtypes = [
+            ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
+            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, MP, ctypes.POINTER(BatchResult)]
+        L.parasail_hip_last_kernel_ms.restype = ctypes.c_double
+        L.parasail_hip_last_error.restype = ctypes.c_char_p
+        L.parasail_hip_available.restype = ctypes.c_int
+        L.parasail_time.restype = ctypes.c_double
+        libc = ctypes.CDLL(None)
+        libc.free.argtypes = [ctypes.c_void_p]
+        L._libc_free = libc.free
+        _lib = L
+    return _lib
+
+
+def available():
+    return bool(lib().parasail_hip_available())
+
+
+def blosum62():
+    return lib().parasail_matrix_lookup(b"blosum62")
+
+
+def matrix_create(alphabet, match, mismatch):
+    return lib().parasail_matrix_create(alphabet.encode(), match, mismatch)
+
+
+def function(name):
+    ptr = lib().parasail_lookup_function(name.encode())
+    if not ptr:
+        raise KeyError(name)
+    return FN(ptr)
+
+
+def _b(s):
+    return s if isinstance(s, bytes) else s.encode()
+
+
+def align(name, s1, s2, open_, gap, matrix):
+    """Call one per-pair drop-in by name; returns a dict with the result
+    fields (and 'cigar' for trace functions)."""
+    f = function(name)
+    a, b = _b(s1) if s1 is not None else None, _b(s2)
+    r = f(a, 0 if a is None else len(a), b, len(b), open_, gap, matrix)
+    if not r:
+        return None
+    res = r.contents
+    out = {"score": res.score, "end_query": res.end_query, "end_ref": res.end_ref,
+           "flag": res.flag}
+    if res.flag & FLAG_STATS:
+        st = ctypes.cast(res.extra, ctypes.POINTER(StatsExtra)).contents
+        out.update(matches=st.matches, similar=st.similar, length=st.length)
+    if res.flag & FLAG_TRACE and not res.flag & FLAG_SATURATED:
+        c = lib().parasail_result_get_cigar(r, a, len(a), b, len(b), matrix)
+        out["cigar"] = cigar_tuple(c)
+    lib().parasail_result_free(r)
+    return out
+
+
+def cigar_tuple(c):
+    """(cigar string, beg_query, beg_ref) from a parasail_cigar_t*, freed."""
+    if not c:
+        return None
+    cc = c.contents
+    ops = [cc.seq[k] for k in range(cc.len)]
+    s = "".join("%d%s" % (v >> 4, "MIDNSHP=XB"[v & 0xF]) for v in ops)
+    out = (s, cc.beg_query, cc.beg_ref)
+    lib().parasail_cigar_free(c)
+    return out
+
+
+def align_batch(alg, kind, width, queries, targets, open_, gap, matrix, want_cigar=False):
+    """Batched alignment through parasail_hip_align_batch.  queries/targets
+    are lists of str/bytes (queries may be None for a PSSM).  Returns a dict
+    of numpy arrays (and a list of cigars for kind 2)."""
+    n = len(targets)
+    tb = [_b(t) for t in targets]
+    tl = np.array([len(t) for t in tb], dtype=np.int32)
+    to = np.zeros(n, dtype=np.int64)
+    if n:
+        to[1:] = np.cumsum(tl, dtype=np.int64)[:-1]
+    tcat = b"".join(tb)
+    if queries is not None:
+        qb = [_b(q) for q in queries]
+        ql = np.array([len(q) for q in qb], dtype=np.int32)
+        qo = np.zeros(n, dtype=np.int64)
+        if n:
+            qo[1:] = np.cumsum(ql, dtype=np.int64)[:-1]
+        qcat = b"".join(qb)
+    res = {k: np.zeros(n, dtype=np.int32) for k in
+           ("score", "end_query", "end_ref", "matches", "similar", "length", "flag")}
+    br = BatchResult()
+    for k in res:
+        setattr(br, k, res[k].ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    cig = None
+    if want_cigar:
+        cig = (ctypes.POINTER(Cigar) * max(n, 1))()
+        br.cigars = ctypes.cast(cig, ctypes.POINTER(ctypes.POINTER(Cigar)))
+    I64 = ctypes.POINTER(ctypes.c_int64)
+    I32 = ctypes.POINTER(ctypes.c_int)
+    rc = lib().parasail_hip_align_batch(
+        alg.encode(), kind, width,
+        qcat if queries is not None else None,
+        qo.ctypes.data_as(I64) if queries is not None else None,
+        ql.ctypes.data_as(I32) if queries is not None else None,
+        tcat, to.ctypes.data_as(I64), tl.ctypes.data_as(I32), n, open_, gap, matrix,
+        ctypes.byref(br))
+    if rc != 0:
+        raise RuntimeError("parasail_hip_align_batch failed: %s"
+                           % lib().parasail_hip_last_error().decode(errors="replace"))
+    if want_cigar:
+        res["cigar"] = [cigar_tuple(cig[k]) for k in range(n)]
+    return res
+
+
+def last_kernel_ms():
+    return lib().parasail_hip_last_kernel_ms()

@@ -1,0 +1,421 @@
+// runtime.hip -- device context, memory pool, resident batches and the
+// phip_* C ABI the host layer calls.
+//
+// One process-wide context (once-guard): device 0, one stream, a pool of
+// device blocks reused across calls.  A resident batch keeps a batch's
+// residues, lengths and results in HBM, and caches one wavefront plan per
+// scoring so that repeated alignments of the same batch launch kernels
+// only.  phip_align (one-shot) is a resident batch created, run and freed.
+// Calls are serialised by one lock: every entry point may be called from
+// any host thread.
+#include <errno.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "wave.h"
+
+static char g_err[512];
+static pthread_once_t g_once = PTHREAD_ONCE_INIT;
+static int g_init_rc = -1;
+static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
+static hipStream_t g_stream;
+static hipEvent_t g_ev0, g_ev1;
+static double g_last_ms;
+
+extern "C" void phip_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *phip_last_error(void) { return g_err; }
+extern "C" double phip_last_kernel_ms(void) { return g_last_ms; }
+
+static void init_once(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        phip_set_error("no HIP device available");
+        g_init_rc = -1;
+        return;
+    }
+    if (hipSetDevice(0) != hipSuccess || hipStreamCreate(&g_stream) != hipSuccess ||
+        hipEventCreate(&g_ev0) != hipSuccess || hipEventCreate(&g_ev1) != hipSuccess) {
+        phip_set_error("HIP context initialisation failed");
+        g_init_rc = -1;
+        return;
+    }
+    g_init_rc = 0;
+}
+
+extern "C" int phip_init(void)
+{
+    pthread_once(&g_once, init_once);
+    if (g_init_rc != 0) errno = ENOSYS;
+    return g_init_rc;
+}
+
+// ---------------------------------------------------------------- pool
+
+namespace {
+
+struct Block {
+    char *p;
+    size_t size;
+    bool used;
+};
+std::vector<Block> g_blocks;
+
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+void *dev_alloc(size_t bytes)
+{
+    bytes = al256(bytes ? bytes : 1);
+    Block *best = nullptr;
+    for (Block &b : g_blocks)
+        if (!b.used && b.size >= bytes && (!best || b.size < best->size)) best = &b;
+    if (best && best->size <= 4 * bytes + ((size_t)16 << 20)) {
+        best->used = true;
+        return best->p;
+    }
+    size_t sz = std::max(bytes, (size_t)1 << 20);
+    char *p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        // release idle blocks and retry once
+        (void)hipStreamSynchronize(g_stream);
+        for (Block &b : g_blocks)
+            if (!b.used) { (void)hipFree(b.p); b.p = nullptr; }
+        g_blocks.erase(std::remove_if(g_blocks.begin(), g_blocks.end(),
+                                      [](const Block &b) { return b.p == nullptr; }),
+                       g_blocks.end());
+        if (hipMalloc(&p, sz) != hipSuccess) {
+            phip_set_error("hipMalloc(%zu) failed", sz);
+            return nullptr;
+        }
+    }
+    g_blocks.push_back(Block{p, sz, true});
+    return p;
+}
+
+void dev_free(void *p)
+{
+    for (Block &b : g_blocks)
+        if (b.p == p) { b.used = false; return; }
+}
+
+// ---------------------------------------------------------------- resident
+
+struct phip_resident {
+    int np;
+    int kind;
+    int has_q;
+    std::vector<int> mvals;
+    phip_matrix_t mx;
+    std::vector<int32_t> qlen, tlen;
+    std::vector<int64_t> qoff, toff, troff;
+    int maxn;
+    char *base;
+    int *dmat;
+    uint8_t *dq, *dt;
+    int64_t *dqoff, *dtoff, *dtroff;
+    int32_t *dqlen, *dtlen;
+    phip_out_t *dout;
+    int8_t *dtr;
+    int64_t trtot;
+    std::vector<WavePlan *> plans;
+};
+
+typedef struct phip_resident phip_resident_t;
+
+static phip_resident_t *resident_create(const phip_batch_t *b)
+{
+    phip_resident_t *r = new phip_resident_t();
+    const int np = b->npairs;
+    r->np = np;
+    r->kind = b->kind;
+    r->has_q = b->q != nullptr;
+    r->mx = b->matrix;
+    r->mvals.assign(b->matrix.values, b->matrix.values + (size_t)b->matrix.size * b->matrix.length);
+    r->mx.values = r->mvals.data();
+    r->qlen.assign(b->qlen, b->qlen + np);
+    r->tlen.assign(b->tlen, b->tlen + np);
+    r->qoff.assign(b->qoff, b->qoff + np);
+    r->toff.assign(b->toff, b->toff + np);
+    int64_t qtot = 0, ttot = 0;
+    r->maxn = 0;
+    r->trtot = 0;
+    r->troff.assign(b->kind == PHIP_TRACE ? np : 0, 0);
+    for (int i = 0; i < np; ++i) {
+        qtot = std::max<int64_t>(qtot, b->qoff[i] + b->qlen[i]);
+        ttot = std::max<int64_t>(ttot, b->toff[i] + b->tlen[i]);
+        r->maxn = std::max(r->maxn, (int)b->tlen[i]);
+        if (b->kind == PHIP_TRACE) {
+            r->troff[i] = r->trtot;
+            r->trtot += (int64_t)b->qlen[i] * b->tlen[i];
+        }
+    }
+    size_t off = 0;
+    auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
+    size_t o_mat = slot(r->mvals.size() * 4), o_q = slot(r->has_q ? qtot : 0), o_t = slot(ttot),
+           o_qo = slot(np * 8), o_to = slot(np * 8), o_ql = slot(np * 4), o_tl = slot(np * 4),
+           o_out = slot(np * sizeof(phip_out_t)), o_tro = slot(r->troff.size() * 8),
+           o_tr = slot(r->trtot);
+    r->base = (char *)dev_alloc(off);
+    if (!r->base) { delete r; return nullptr; }
+    r->dmat = (int *)(r->base + o_mat);
+    r->dq = r->has_q ? (uint8_t *)(r->base + o_q) : nullptr;
+    r->dt = (uint8_t *)(r->base + o_t);
+    r->dqoff = (int64_t *)(r->base + o_qo);
+    r->dtoff = (int64_t *)(r->base + o_to);
+    r->dqlen = (int32_t *)(r->base + o_ql);
+    r->dtlen = (int32_t *)(r->base + o_tl);
+    r->dout = (phip_out_t *)(r->base + o_out);
+    r->dtroff = (int64_t *)(r->base + o_tro);
+    r->dtr = b->kind == PHIP_TRACE ? (int8_t *)(r->base + o_tr) : nullptr;
+    bool ok = true;
+    ok &= hipMemcpyAsync(r->dmat, r->mvals.data(), r->mvals.size() * 4, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+    if (r->has_q) ok &= hipMemcpyAsync(r->dq, b->q, qtot, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+    ok &= hipMemcpyAsync(r->dt, b->t, ttot, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+    ok &= hipMemcpyAsync(r->dqoff, b->qoff, np * 8, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+    ok &= hipMemcpyAsync(r->dtoff, b->toff, np * 8, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+    ok &= hipMemcpyAsync(r->dqlen, b->qlen, np * 4, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+    ok &= hipMemcpyAsync(r->dtlen, b->tlen, np * 4, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+    if (!r->troff.empty())
+        ok &= hipMemcpyAsync(r->dtroff, r->troff.data(), np * 8, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+    ok &= hipStreamSynchronize(g_stream) == hipSuccess;
+    if (!ok) {
+        phip_set_error("resident batch upload failed");
+        dev_free(r->base);
+        delete r;
+        return nullptr;
+    }
+    return r;
+}
+
+static void resident_free(phip_resident_t *r)
+{
+    if (!r) return;
+    for (WavePlan *p : r->plans) wave_plan_free(p);
+    dev_free(r->base);
+    delete r;
+}
+
+static DPParams make_params(const phip_resident_t *r, int alg, int kind, int sg_flags, int open,
+                            int gap, int check)
+{
+    DPParams dp;
+    memset(&dp, 0, sizeof(dp));
+    dp.alg = alg;
+    dp.kind = kind;
+    dp.sg_flags = alg == PHIP_SG ? sg_flags : 0;
+    dp.open = open;
+    dp.gap = gap;
+    dp.check = check;
+    dp.matrix = r->dmat;
+    dp.msize = r->mx.size;
+    dp.is_pssm = r->mx.is_pssm;
+    dp.q = r->dq;
+    dp.qoff = r->dqoff;
+    dp.qlen = r->dqlen;
+    dp.t = r->dt;
+    dp.toff = r->dtoff;
+    dp.tlen = r->dtlen;
+    dp.npairs = r->np;
+    dp.out = r->dout;
+    dp.trace = r->dtr;
+    dp.trace_off = r->dtroff;
+    return dp;
+}
+
+// Runs one alignment pass over the resident batch; results stay in r->dout.
+static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int open, int gap,
+                        int check)
+{
+    DPParams dp = make_params(r, alg, kind, sg_flags, open, gap, check);
+    const bool wave = kind == PHIP_SCORE && !r->mx.is_pssm && !(check && alg != PHIP_SW);
+    if (wave) {
+        WavePlan *plan = nullptr;
+        for (WavePlan *p : r->plans)
+            if (wave_plan_matches(p, alg, dp.sg_flags, open, gap)) plan = p;
+        if (!plan) {
+            HostBatch hb;
+            hb.np = r->np;
+            hb.qlen = r->qlen.data();
+            hb.tlen = r->tlen.data();
+            hb.qoff = r->qoff.data();
+            hb.toff = r->toff.data();
+            hb.mx = &r->mx;
+            plan = wave_plan_create(hb, dp, g_stream);
+            if (!plan) return -1;
+            r->plans.push_back(plan);
+        }
+        HIP_OK(hipEventRecord(g_ev0, g_stream));
+        if (wave_plan_run(plan, dp, g_stream) != 0) return -1;
+    } else {
+        const int narr = kind == PHIP_STATS ? 8 : 2;
+        const size_t per_lane = (size_t)(r->maxn + 1) * narr * sizeof(int);
+        const int lanes = (int)std::max<size_t>(
+            64, std::min<size_t>((size_t)r->np, ((size_t)1 << 30) / per_lane));
+        int *scratch = (int *)dev_alloc(per_lane * lanes);
+        if (!scratch) return -1;
+        HIP_OK(hipEventRecord(g_ev0, g_stream));
+        for (int first = 0; first < r->np; first += lanes) {
+            const int count = std::min(lanes, r->np - first);
+            hipLaunchKernelGGL(k_simple, dim3((count + 255) / 256), dim3(256), 0, g_stream, dp,
+                               first, count, (const int32_t *)nullptr, (const int *)nullptr,
+                               scratch, count, r->maxn);
+            if (hipGetLastError() != hipSuccess) {
+                dev_free(scratch);
+                phip_set_error("k_simple launch failed");
+                return -1;
+            }
+        }
+        // the block goes back to the pool; later users queue behind these
+        // kernels on the same stream
+        dev_free(scratch);
+    }
+    HIP_OK(hipEventRecord(g_ev1, g_stream));
+    return 0;
+}
+
+static int resident_fetch(phip_resident_t *r, phip_out_t *out, int8_t **traces)
+{
+    if (out)
+        HIP_OK(hipMemcpyAsync(out, r->dout, (size_t)r->np * sizeof(phip_out_t),
+                              hipMemcpyDeviceToHost, g_stream));
+    HIP_OK(hipStreamSynchronize(g_stream));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
+    g_last_ms = ms;
+    if (traces && r->dtr) {
+        for (int i = 0; i < r->np; ++i) {
+            if (!traces[i]) continue;
+            HIP_OK(hipMemcpy(traces[i], r->dtr + r->troff[i], (size_t)r->qlen[i] * r->tlen[i],
+                             hipMemcpyDeviceToHost));
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **traces)
+{
+    if (phip_init() != 0) return -1;
+    if (!b || !out || b->npairs < 0) { phip_set_error("phip_align: bad arguments"); return -1; }
+    if (b->npairs == 0) return 0;
+    pthread_mutex_lock(&g_lock);
+    int rc = -1;
+    phip_resident_t *r = resident_create(b);
+    if (r) {
+        rc = resident_run(r, b->alg, b->kind, b->sg_flags, b->open, b->gap, b->want_sat_check);
+        if (rc == 0) rc = resident_fetch(r, out, traces);
+        resident_free(r);
+    }
+    pthread_mutex_unlock(&g_lock);
+    return rc;
+}
+
+extern "C" void *phip_resident_create(const phip_batch_t *b)
+{
+    if (phip_init() != 0) return nullptr;
+    if (!b || b->npairs <= 0) { phip_set_error("phip_resident_create: bad arguments"); return nullptr; }
+    pthread_mutex_lock(&g_lock);
+    phip_resident_t *r = resident_create(b);
+    pthread_mutex_unlock(&g_lock);
+    return r;
+}
+
+extern "C" int phip_resident_align(void *handle, int alg, int sg_flags, int open, int gap,
+                                   phip_out_t *out)
+{
+    phip_resident_t *r = (phip_resident_t *)handle;
+    if (!r) { phip_set_error("phip_resident_align: no batch"); return -1; }
+    pthread_mutex_lock(&g_lock);
+    int rc = resident_run(r, alg, PHIP_SCORE, sg_flags, open, gap, 0);
+    if (rc == 0) rc = resident_fetch(r, out, nullptr);
+    pthread_mutex_unlock(&g_lock);
+    return rc;
+}
+
+extern "C" void phip_resident_free(void *handle)
+{
+    pthread_mutex_lock(&g_lock);
+    resident_free((phip_resident_t *)handle);
+    pthread_mutex_unlock(&g_lock);
+}
+
+// ---------------------------------------------------------------- banded
+
+extern "C" __global__ void k_banded(const uint8_t *s1, int m, const uint8_t *s2, int n, int open,
+                                    int gap, int k, const int *mat, int msize, int is_pssm,
+                                    int *HB, int *EB, phip_out_t *out);
+
+static int banded_locked(const uint8_t *s1, int m, const uint8_t *s2, int n, int open, int gap,
+                         int k, const phip_matrix_t *mx, phip_out_t *out)
+{
+    const int low = (m > n) ? n - m - k : m - n - k;
+    const size_t colLen = (size_t)(k - low + 1);
+    const size_t mbytes = (size_t)mx->size * mx->length * sizeof(int);
+    size_t off = 0;
+    auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes); return o; };
+    size_t o_mat = slot(mbytes), o_1 = slot(m), o_2 = slot(n), o_h = slot((colLen + 2) * 4),
+           o_e = slot((colLen + 2) * 4), o_out = slot(sizeof(phip_out_t));
+    char *base = (char *)dev_alloc(off);
+    if (!base) return -1;
+    int rc = -1;
+    do {
+        HIP_OK(hipMemcpyAsync(base + o_mat, mx->values, mbytes, hipMemcpyHostToDevice, g_stream));
+        HIP_OK(hipMemcpyAsync(base + o_1, s1, m, hipMemcpyHostToDevice, g_stream));
+        HIP_OK(hipMemcpyAsync(base + o_2, s2, n, hipMemcpyHostToDevice, g_stream));
+        HIP_OK(hipEventRecord(g_ev0, g_stream));
+        hipLaunchKernelGGL(k_banded, dim3(1), dim3(64), 0, g_stream, (const uint8_t *)(base + o_1),
+                           m, (const uint8_t *)(base + o_2), n, open, gap, k,
+                           (const int *)(base + o_mat), mx->size, mx->is_pssm,
+                           (int *)(base + o_h), (int *)(base + o_e), (phip_out_t *)(base + o_out));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(g_ev1, g_stream));
+        HIP_OK(hipMemcpyAsync(out, base + o_out, sizeof(phip_out_t), hipMemcpyDeviceToHost, g_stream));
+        HIP_OK(hipStreamSynchronize(g_stream));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
+        g_last_ms = ms;
+        rc = 0;
+    } while (0);
+    dev_free(base);
+    return rc;
+}
+
+extern "C" int phip_nw_banded(const uint8_t *s1, int m, const uint8_t *s2, int n, int open, int gap,
+                              int k, const phip_matrix_t *mx, phip_out_t *out)
+{
+    if (phip_init() != 0) return -1;
+    if (!s1 || !s2 || m <= 0 || n <= 0 || k <= 0 || !mx || !out) {
+        phip_set_error("phip_nw_banded: bad arguments");
+        return -1;
+    }
+    pthread_mutex_lock(&g_lock);
+    int rc = banded_locked(s1, m, s2, n, open, gap, k, mx, out);
+    pthread_mutex_unlock(&g_lock);
+    return rc;
+}
+
+extern "C" int phip_device_info(int *cus, int *clock_khz)
+{
+    if (phip_init() != 0) return -1;
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, 0));
+    if (cus) *cus = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    return 0;
+}

@@ -1,0 +1,31 @@
+// wave.h -- packed-int16 wavefront tier (score-only SW / NW / SG): the
+// device-resident plan that binds a batch of pairs to the kernels.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+// Host view of a batch (lengths and offsets) used for planning.
+struct HostBatch {
+    int np;
+    const int32_t *qlen, *tlen;
+    const int64_t *qoff, *toff;
+    const phip_matrix_t *mx;
+};
+
+// Device pool: reuses blocks across calls so per-call entry points do not
+// pay hipMalloc/hipFree.
+void *dev_alloc(size_t bytes);
+void dev_free(void *p);
+
+struct WavePlan;
+
+// Plans and uploads everything that depends only on the batch and the
+// scoring (jobs, column streams, LDS table, output slots).  Pairs that do
+// not provably fit int16 are listed for the int32 tier.  NULL on failure.
+WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t s);
+// Launches the plan asynchronously on s; results land in dp.out.
+int wave_plan_run(WavePlan *p, const DPParams &dp, hipStream_t s);
+void wave_plan_free(WavePlan *p);
+bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap);

@@ -1,0 +1,412 @@
+/*
+ * align.c -- the hip family's per-call entry points.
+ *
+ * Each parasail_<alg>[_stats|_trace]_<strategy>[_profile]_hip_64_<width>
+ * validates its arguments the way the reference does (PARASAIL_CHECK_*,
+ * parasail/memory.h:47-94), maps the residues through matrix->mapper, runs
+ * one pair through the device (phip_align) and returns an ordinary
+ * parasail_result_t.
+ *
+ * Widths.  The device computes every pair exactly (packed int16 when the
+ * pair provably fits, int32 otherwise).  A width only decides how the
+ * result is reported: a width whose window the exact DP values leave is
+ * reported saturated with the reference's saturated outputs (SW: score
+ * 2^w-1 and ends 0, src/sw_striped_avx2_256_16.c:374-382; NW/SG: score 0,
+ * ends 0, src/nw_striped_avx2_256_16.c:380-385), and "sat" reports the
+ * narrowest width that holds the alignment (src/satcheck.c:41520-41537).
+ * The windows are the reference's: SW saturates past 2^w - max - 2
+ * (maxp = INT_MAX(w) - (max+1) on the biased scale), NW/SG when a value
+ * falls below NEG_LIMIT or H exceeds POS_LIMIT
+ * (src/sg_striped_avx2_256_16.c:209-210).
+ */
+#include <errno.h>
+#include <limits.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+static int width_max(int w) { return w == 8 ? INT8_MAX : w == 16 ? INT16_MAX : INT32_MAX; }
+static int width_min(int w) { return w == 8 ? INT8_MIN : w == 16 ? INT16_MIN : INT32_MIN; }
+
+int phost_saturates(int alg, int w, int open, const parasail_matrix_t *matrix, const phip_out_t *o)
+{
+    if (w >= 32) return 0;
+    if (alg == PHIP_SW) {
+        long limit = (1L << w) - matrix->max - 2;
+        return o->score > limit;
+    }
+    long neg = (-open < matrix->min ? (long)width_min(w) + open : (long)width_min(w) - matrix->min) + 1;
+    long pos = (long)width_max(w) - matrix->max - 1;
+    return o->min_val < neg || o->max_val > pos;
+}
+
+static int bits_flag(int w)
+{
+    return w == 8 ? PARASAIL_FLAG_BITS_8 : w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32;
+}
+
+#define CHECK_NULL(fn, v)                                        \
+    do {                                                         \
+        if (!(v)) {                                              \
+            fprintf(stderr, "%s: missing %s\n", fn, #v);         \
+            return NULL;                                         \
+        }                                                        \
+    } while (0)
+#define CHECK_GT0(fn, v)                                         \
+    do {                                                         \
+        if ((v) <= 0) {                                          \
+            fprintf(stderr, "%s: %s must be > 0\n", fn, #v);     \
+            return NULL;                                         \
+        }                                                        \
+    } while (0)
+#define CHECK_GE0(fn, v)                                         \
+    do {                                                         \
+        if ((v) < 0) {                                           \
+            fprintf(stderr, "%s: %s must be >= 0\n", fn, #v);    \
+            return NULL;                                         \
+        }                                                        \
+    } while (0)
+
+static int sg_result_flags(int sg_flags)
+{
+    int f = 0;
+    if (sg_flags & PHIP_S1_BEG) f |= PARASAIL_FLAG_SG_S1_BEG;
+    if (sg_flags & PHIP_S1_END) f |= PARASAIL_FLAG_SG_S1_END;
+    if (sg_flags & PHIP_S2_BEG) f |= PARASAIL_FLAG_SG_S2_BEG;
+    if (sg_flags & PHIP_S2_END) f |= PARASAIL_FLAG_SG_S2_END;
+    return f;
+}
+
+parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kind, int width,
+                               int scan, const char *s1, int s1Len, const char *s2, int s2Len,
+                               int open, int gap, const parasail_matrix_t *matrix,
+                               const parasail_profile_t *profile)
+{
+    (void)profile;
+    CHECK_NULL(fname, s2);
+    CHECK_GT0(fname, s2Len);
+    CHECK_GE0(fname, open);
+    CHECK_GE0(fname, gap);
+    CHECK_NULL(fname, matrix);
+    const int pssm = matrix->type == PARASAIL_MATRIX_TYPE_PSSM;
+    if (!pssm) {
+        CHECK_NULL(fname, s1);
+        CHECK_GT0(fname, s1Len);
+    }
+    if (kind == PHIP_STATS && !s1) {
+        fprintf(stderr, "%s: stats functions with pssm require a consensus sequence\n", fname);
+        return NULL;
+    }
+    if (matrix->size > 256 || matrix->size <= 0) {
+        fprintf(stderr, "%s: matrix size %d unsupported\n", fname, matrix->size);
+        return NULL;
+    }
+    if (phip_init() != 0) {
+        errno = ENOSYS;
+        return NULL;
+    }
+    const int m = pssm ? matrix->length : s1Len;
+    const int mq = s1 ? (pssm && s1Len > 0 ? s1Len : m) : 0;
+    if (pssm && s1 && mq < m) {
+        fprintf(stderr, "%s: consensus sequence shorter than the pssm\n", fname);
+        return NULL;
+    }
+    uint8_t *q = NULL, *t = (uint8_t *)malloc((size_t)s2Len);
+    if (s1) q = (uint8_t *)malloc((size_t)m);
+    if (!t || (s1 && !q)) { free(q); free(t); return NULL; }
+    for (int i = 0; s1 && i < m; ++i) q[i] = (uint8_t)matrix->mapper[(unsigned char)s1[i]];
+    for (int j = 0; j < s2Len; ++j) t[j] = (uint8_t)matrix->mapper[(unsigned char)s2[j]];
+
+    phip_batch_t b;
+    memset(&b, 0, sizeof(b));
+    b.alg = alg;
+    b.kind = kind;
+    b.sg_flags = sg_flags;
+    b.open = open;
+    b.gap = gap;
+    b.want_sat_check = alg != PHIP_SW && width != 32;
+    b.matrix.values = matrix->matrix;
+    b.matrix.size = matrix->size;
+    b.matrix.length = pssm ? matrix->length : matrix->size;
+    b.matrix.is_pssm = pssm;
+    b.matrix.max = matrix->max;
+    b.matrix.min = matrix->min;
+    b.npairs = 1;
+    int64_t qo = 0, to = 0;
+    int32_t ql = m, tl = s2Len;
+    b.q = q;
+    b.qoff = &qo;
+    b.qlen = &ql;
+    b.t = t;
+    b.toff = &to;
+    b.tlen = &tl;
+    phip_out_t o;
+    int8_t *tr = NULL;
+    if (kind == PHIP_TRACE) {
+        tr = (int8_t *)malloc((size_t)m * s2Len);
+        if (!tr) { free(q); free(t); return NULL; }
+    }
+    int rc = phip_align(&b, &o, kind == PHIP_TRACE ? &tr : NULL);
+    free(q);
+    free(t);
+    if (rc != 0) {
+        fprintf(stderr, "%s: %s\n", fname, phip_last_error());
+        free(tr);
+        return NULL;
+    }
+
+    int w = width;
+    int saturated = 0;
+    if (w == 0) {
+        w = !phost_saturates(alg, 8, open, matrix, &o) ? 8
+            : !phost_saturates(alg, 16, open, matrix, &o) ? 16 : 32;
+    } else {
+        saturated = phost_saturates(alg, w, open, matrix, &o);
+    }
+
+    parasail_result_t *r = NULL;
+    if (kind == PHIP_STATS) {
+        r = phost_result_new_stats();
+    } else if (kind == PHIP_TRACE) {
+        const int lanes = 64, seglen = (m + lanes - 1) / lanes;
+        const size_t esz = w == 8 ? 1 : w == 16 ? 2 : 4;
+        r = phost_result_new_trace(seglen * lanes, s2Len, 64, esz);
+        if (r) {
+            /* striped layout of cigar.c:157 with 64 segments */
+            memset(r->trace->trace_table, 0, esz * (size_t)seglen * lanes * s2Len);
+            for (long j = 0; j < s2Len; ++j)
+                for (long i = 0; i < m; ++i) {
+                    long loc = j * seglen * lanes + (i % seglen) * lanes + i / seglen;
+                    int v = tr[i * s2Len + j];
+                    if (esz == 1) ((int8_t *)r->trace->trace_table)[loc] = (int8_t)v;
+                    else if (esz == 2) ((int16_t *)r->trace->trace_table)[loc] = (int16_t)v;
+                    else ((int32_t *)r->trace->trace_table)[loc] = v;
+                }
+        }
+    } else {
+        r = phost_result_new();
+    }
+    free(tr);
+    if (!r) return NULL;
+    r->flag |= alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
+    if (alg == PHIP_SG) r->flag |= sg_result_flags(sg_flags);
+    r->flag |= scan ? PARASAIL_FLAG_SCAN : PARASAIL_FLAG_STRIPED;
+    r->flag |= bits_flag(w) | PARASAIL_FLAG_LANES_64;
+    if (kind == PHIP_STATS) r->flag |= PARASAIL_FLAG_STATS;
+    if (kind == PHIP_TRACE) r->flag |= PARASAIL_FLAG_TRACE;
+    if (saturated) {
+        r->flag |= PARASAIL_FLAG_SATURATED;
+        r->score = alg == PHIP_SW ? (int)((1L << w) - 1) : 0;
+        r->end_query = 0;
+        r->end_ref = 0;
+        if (kind == PHIP_STATS) {
+            r->stats->matches = 0;
+            r->stats->similar = 0;
+            r->stats->length = 0;
+        }
+    } else {
+        r->score = o.score;
+        r->end_query = o.end_query;
+        r->end_ref = o.end_ref;
+        if (kind == PHIP_STATS) {
+            r->stats->matches = o.matches;
+            r->stats->similar = o.similar;
+            r->stats->length = o.length;
+        }
+    }
+    return r;
+}
+
+static parasail_result_t *phost_align_profile(const char *fname, int alg, int sg_flags, int kind,
+                                              int width, int scan,
+                                              const parasail_profile_t *profile, const char *s2,
+                                              int s2Len, int open, int gap)
+{
+    CHECK_NULL(fname, profile);
+    CHECK_NULL(fname, profile->matrix);
+    CHECK_GT0(fname, profile->s1Len);
+    return phost_align(fname, alg, sg_flags, kind, width, scan, profile->s1, profile->s1Len, s2,
+                       s2Len, open, gap, profile->matrix, profile);
+}
+
+/* ---- the generated entry points ---- */
+
+#define ALG_sw PHIP_SW
+#define ALG_nw PHIP_NW
+#define ALG_sg PHIP_SG
+#define ALG_sg_qb PHIP_SG
+#define ALG_sg_qe PHIP_SG
+#define ALG_sg_qx PHIP_SG
+#define ALG_sg_db PHIP_SG
+#define ALG_sg_de PHIP_SG
+#define ALG_sg_dx PHIP_SG
+#define ALG_sg_qb_de PHIP_SG
+#define ALG_sg_qe_db PHIP_SG
+#define ALG_sg_qb_db PHIP_SG
+#define ALG_sg_qe_de PHIP_SG
+/* src/sg_helper.h:53-63: (s1_beg, s1_end, s2_beg, s2_end) per name */
+#define SGF_sw 0
+#define SGF_nw 0
+#define SGF_sg (PHIP_S1_BEG | PHIP_S1_END | PHIP_S2_BEG | PHIP_S2_END)
+#define SGF_sg_qb (PHIP_S1_BEG)
+#define SGF_sg_qe (PHIP_S1_END)
+#define SGF_sg_qx (PHIP_S1_BEG | PHIP_S1_END)
+#define SGF_sg_db (PHIP_S2_BEG)
+#define SGF_sg_de (PHIP_S2_END)
+#define SGF_sg_dx (PHIP_S2_BEG | PHIP_S2_END)
+#define SGF_sg_qb_de (PHIP_S1_BEG | PHIP_S2_END)
+#define SGF_sg_qe_db (PHIP_S1_END | PHIP_S2_BEG)
+#define SGF_sg_qb_db (PHIP_S1_BEG | PHIP_S2_BEG)
+#define SGF_sg_qe_de (PHIP_S1_END | PHIP_S2_END)
+
+#define KIND_ PHIP_SCORE
+#define KIND__stats PHIP_STATS
+#define KIND__trace PHIP_TRACE
+#define SCAN_striped 0
+#define SCAN_scan 1
+#define WID_8 8
+#define WID_16 16
+#define WID_32 32
+#define WID_sat 0
+
+#define DEF_W(alg, kind, strat, w)                                                          \
+    PHOST_EXPORT parasail_result_t *parasail_##alg##kind##_##strat##_hip_64_##w(            \
+        const char *const s1, const int s1Len, const char *const s2, const int s2Len,        \
+        const int open, const int gap, const parasail_matrix_t *matrix)                      \
+    {                                                                                        \
+        return phost_align(__func__, ALG_##alg, SGF_##alg, KIND_##kind, WID_##w,             \
+                           SCAN_##strat, s1, s1Len, s2, s2Len, open, gap, matrix, NULL);     \
+    }                                                                                        \
+    PHOST_EXPORT parasail_result_t *parasail_##alg##kind##_##strat##_profile_hip_64_##w(    \
+        const parasail_profile_t *const profile, const char *const s2, const int s2Len,     \
+        const int open, const int gap)                                                       \
+    {                                                                                        \
+        return phost_align_profile(__func__, ALG_##alg, SGF_##alg, KIND_##kind, WID_##w,     \
+                                   SCAN_##strat, profile, s2, s2Len, open, gap);             \
+    }
+
+#define DEF_S(alg, kind, strat) \
+    DEF_W(alg, kind, strat, 8) DEF_W(alg, kind, strat, 16) DEF_W(alg, kind, strat, 32) DEF_W(alg, kind, strat, sat)
+
+#define DEF_ALG(alg)                                                                        \
+    DEF_S(alg, , striped) DEF_S(alg, , scan) DEF_S(alg, _stats, striped)                    \
+    DEF_S(alg, _stats, scan) DEF_S(alg, _trace, striped) DEF_S(alg, _trace, scan)
+
+PARASAIL_HIP_ALGS(DEF_ALG)
+
+/* ---- profiles ---- */
+
+static void profile_data_free(void *p) { free(p); }
+
+/* The hip profile keeps the query's residue codes; the substitution table
+ * travels with every launch (src/memory_avx2.c:81-132 builds striped score
+ * vectors instead, which the wavefront kernels do not need). */
+static parasail_profile_t *profile_create(const char *fname, const char *s1, int s1Len,
+                                          const parasail_matrix_t *matrix, int stats, int w)
+{
+    CHECK_NULL(fname, matrix);
+    if (matrix->type == PARASAIL_MATRIX_TYPE_SQUARE) {
+        CHECK_NULL(fname, s1);
+        CHECK_GT0(fname, s1Len);
+    }
+    if (stats && !s1) {
+        fprintf(stderr, "%s: stats functions with pssm require a consensus sequence\n", fname);
+        return NULL;
+    }
+    if (phip_init() != 0) {
+        errno = ENOSYS;
+        return NULL;
+    }
+    parasail_profile_t *p = phost_profile_new(s1, s1Len, matrix);
+    if (!p) return NULL;
+    const int m = p->s1Len;
+    uint8_t *codes = (uint8_t *)malloc((size_t)m + 1);
+    if (!codes) { free(p); return NULL; }
+    for (int i = 0; i < m; ++i) codes[i] = s1 ? (uint8_t)matrix->mapper[(unsigned char)s1[i]] : 0;
+    p->free = profile_data_free;
+    struct parasail_profile_data *d = w == 8 ? &p->profile8 : w == 32 ? &p->profile32 : &p->profile16;
+    d->score = codes;
+    if (w == 0) {
+        /* _sat carries all widths (src/memory_avx2.c:240-253) */
+        p->profile8.score = NULL;
+        p->profile16.score = codes;
+    }
+    return p;
+}
+
+#define DEF_PC(w, W)                                                                         \
+    PHOST_EXPORT parasail_profile_t *parasail_profile_create_hip_64_##w(                     \
+        const char *const s1, const int s1Len, const parasail_matrix_t *matrix)               \
+    {                                                                                         \
+        return profile_create(__func__, s1, s1Len, matrix, 0, W);                            \
+    }                                                                                         \
+    PHOST_EXPORT parasail_profile_t *parasail_profile_create_stats_hip_64_##w(               \
+        const char *const s1, const int s1Len, const parasail_matrix_t *matrix)               \
+    {                                                                                         \
+        return profile_create(__func__, s1, s1Len, matrix, 1, W);                            \
+    }
+DEF_PC(8, 8)
+DEF_PC(16, 16)
+DEF_PC(32, 32)
+DEF_PC(sat, 0)
+
+/* ---- banded NW ---- */
+
+PHOST_EXPORT parasail_result_t *parasail_nw_banded_hip(const char *const s1, const int s1Len,
+                                                       const char *const s2, const int s2Len,
+                                                       const int open, const int gap, const int k,
+                                                       const parasail_matrix_t *matrix)
+{
+    const char *fname = __func__;
+    CHECK_NULL(fname, s1);
+    CHECK_GT0(fname, s1Len);
+    CHECK_NULL(fname, s2);
+    CHECK_GT0(fname, s2Len);
+    CHECK_GE0(fname, open);
+    CHECK_GE0(fname, gap);
+    CHECK_GT0(fname, k);
+    CHECK_NULL(fname, matrix);
+    if (phip_init() != 0) {
+        errno = ENOSYS;
+        return NULL;
+    }
+    uint8_t *q = (uint8_t *)malloc((size_t)s1Len), *t = (uint8_t *)malloc((size_t)s2Len);
+    if (!q || !t) { free(q); free(t); return NULL; }
+    for (int i = 0; i < s1Len; ++i) q[i] = (uint8_t)matrix->mapper[(unsigned char)s1[i]];
+    for (int j = 0; j < s2Len; ++j) t[j] = (uint8_t)matrix->mapper[(unsigned char)s2[j]];
+    phip_matrix_t mx;
+    mx.values = matrix->matrix;
+    mx.size = matrix->size;
+    mx.length = matrix->type == PARASAIL_MATRIX_TYPE_PSSM ? matrix->length : matrix->size;
+    mx.is_pssm = matrix->type == PARASAIL_MATRIX_TYPE_PSSM;
+    mx.max = matrix->max;
+    mx.min = matrix->min;
+    phip_out_t o;
+    int rc = phip_nw_banded(q, s1Len, t, s2Len, open, gap, k, &mx, &o);
+    free(q);
+    free(t);
+    if (rc != 0) {
+        fprintf(stderr, "%s: %s\n", fname, phip_last_error());
+        return NULL;
+    }
+    parasail_result_t *r = phost_result_new();
+    if (!r) return NULL;
+    /* src/nw_banded.c:75-77 */
+    r->flag |= PARASAIL_FLAG_NW | PARASAIL_FLAG_BANDED | PARASAIL_FLAG_BITS_32 | PARASAIL_FLAG_LANES_64;
+    r->score = o.score;
+    r->end_query = s1Len - 1;
+    r->end_ref = s2Len - 1;
+    return r;
+}
+
+/* ---- misc ---- */
+
+PHOST_EXPORT double parasail_hip_last_kernel_ms(void) { return phip_last_kernel_ms(); }
+PHOST_EXPORT const char *parasail_hip_last_error(void) { return phip_last_error(); }
+PHOST_EXPORT int parasail_hip_available(void) { return phip_init() == 0; }
+PHOST_EXPORT int parasail_hip_device_info(int *compute_units, int *clock_khz)
+{
+    return phip_device_info(compute_units, clock_khz);
+}

@@ -1,0 +1,326 @@
+/*
+ * batch.c -- parasail_hip_align_batch: many pairs in one device pass.
+ *
+ * The reference has no batched entry point; its callers loop over pairs
+ * with OpenMP (tests/test_openmp.c:383-441, apps/parasail_aligner.cpp:
+ * 1606-1625).  Here the loop moves onto the device: one call maps the
+ * residues, ships the batch to HBM once and aligns every pair there.
+ * Results are the ones the per-pair hip functions return, in flat arrays.
+ */
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+static int parse_alg(const char *name, int *alg, int *flags)
+{
+    static const struct { const char *n; int alg, flags; } tab[] = {
+        {"sw", PHIP_SW, 0},
+        {"nw", PHIP_NW, 0},
+        {"sg", PHIP_SG, PHIP_S1_BEG | PHIP_S1_END | PHIP_S2_BEG | PHIP_S2_END},
+        {"sg_qb", PHIP_SG, PHIP_S1_BEG},
+        {"sg_qe", PHIP_SG, PHIP_S1_END},
+        {"sg_qx", PHIP_SG, PHIP_S1_BEG | PHIP_S1_END},
+        {"sg_db", PHIP_SG, PHIP_S2_BEG},
+        {"sg_de", PHIP_SG, PHIP_S2_END},
+        {"sg_dx", PHIP_SG, PHIP_S2_BEG | PHIP_S2_END},
+        {"sg_qb_de", PHIP_SG, PHIP_S1_BEG | PHIP_S2_END},
+        {"sg_qe_db", PHIP_SG, PHIP_S1_END | PHIP_S2_BEG},
+        {"sg_qb_db", PHIP_SG, PHIP_S1_BEG | PHIP_S2_BEG},
+        {"sg_qe_de", PHIP_SG, PHIP_S1_END | PHIP_S2_END},
+    };
+    for (size_t i = 0; i < sizeof(tab) / sizeof(tab[0]); ++i)
+        if (strcmp(name, tab[i].n) == 0) {
+            *alg = tab[i].alg;
+            *flags = tab[i].flags;
+            return 0;
+        }
+    return -1;
+}
+
+static char g_err[256];
+
+PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int width,
+                                          const char *s1, const int64_t *s1_off,
+                                          const int *s1_len, const char *s2,
+                                          const int64_t *s2_off, const int *s2_len, int npairs,
+                                          int open, int gap, const parasail_matrix_t *matrix,
+                                          parasail_hip_batch_result_t *out)
+{
+    int alg, flags;
+    if (!alg_name || parse_alg(alg_name, &alg, &flags) != 0) {
+        fprintf(stderr, "%s: unknown algorithm %s\n", __func__, alg_name ? alg_name : "(null)");
+        return -1;
+    }
+    if (!matrix || !s2 || !s2_off || !s2_len || npairs < 0 || open < 0 || gap < 0 || !out ||
+        kind < 0 || kind > 2 || (width != 0 && width != 8 && width != 16 && width != 32)) {
+        fprintf(stderr, "%s: invalid arguments\n", __func__);
+        return -1;
+    }
+    const int pssm = matrix->type == PARASAIL_MATRIX_TYPE_PSSM;
+    if (!pssm && (!s1 || !s1_off || !s1_len)) {
+        fprintf(stderr, "%s: missing s1\n", __func__);
+        return -1;
+    }
+    if (kind == PHIP_STATS && !s1) {
+        fprintf(stderr, "%s: stats functions with pssm require a consensus sequence\n", __func__);
+        return -1;
+    }
+    if (phip_init() != 0) {
+        errno = ENOSYS;
+        return -1;
+    }
+    if (npairs == 0) return 0;
+    int64_t qtot = 0, ttot = 0;
+    int32_t *ql = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
+    int32_t *tl = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
+    int64_t *qo = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
+    int64_t *to = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
+    phip_out_t *res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)npairs);
+    if (!ql || !tl || !qo || !to || !res) goto oom;
+    for (int k = 0; k < npairs; ++k) {
+        ql[k] = pssm ? matrix->length : s1_len[k];
+        tl[k] = s2_len[k];
+        if (ql[k] <= 0 || tl[k] <= 0) {
+            fprintf(stderr, "%s: pair %d has an empty sequence\n", __func__, k);
+            free(ql); free(tl); free(qo); free(to); free(res);
+            return -1;
+        }
+        qo[k] = qtot;
+        to[k] = ttot;
+        qtot += s1 ? ql[k] : 0;
+        ttot += tl[k];
+    }
+    {
+        uint8_t *q = s1 ? (uint8_t *)malloc((size_t)qtot + 1) : NULL;
+        uint8_t *t = (uint8_t *)malloc((size_t)ttot + 1);
+        int8_t **tr = NULL;
+        if ((s1 && !q) || !t) { free(q); free(t); goto oom; }
+        for (int k = 0; k < npairs; ++k) {
+            if (s1) {
+                const char *src = s1 + s1_off[k];
+                for (int i = 0; i < ql[k]; ++i) q[qo[k] + i] = (uint8_t)matrix->mapper[(unsigned char)src[i]];
+            }
+            const char *src = s2 + s2_off[k];
+            for (int j = 0; j < tl[k]; ++j) t[to[k] + j] = (uint8_t)matrix->mapper[(unsigned char)src[j]];
+        }
+        if (kind == PHIP_TRACE && out->cigars) {
+            tr = (int8_t **)calloc((size_t)npairs, sizeof(int8_t *));
+            if (!tr) { free(q); free(t); goto oom; }
+            for (int k = 0; k < npairs; ++k) {
+                tr[k] = (int8_t *)malloc((size_t)ql[k] * tl[k]);
+                if (!tr[k]) {
+                    for (int z = 0; z < k; ++z) free(tr[z]);
+                    free(tr); free(q); free(t);
+                    goto oom;
+                }
+            }
+        }
+        phip_batch_t b;
+        memset(&b, 0, sizeof(b));
+        b.alg = alg;
+        b.kind = kind;
+        b.sg_flags = flags;
+        b.open = open;
+        b.gap = gap;
+        b.want_sat_check = alg != PHIP_SW && (width == 8 || width == 16);
+        b.matrix.values = matrix->matrix;
+        b.matrix.size = matrix->size;
+        b.matrix.length = pssm ? matrix->length : matrix->size;
+        b.matrix.is_pssm = pssm;
+        b.matrix.max = matrix->max;
+        b.matrix.min = matrix->min;
+        b.npairs = npairs;
+        b.q = q;
+        b.qoff = qo;
+        b.qlen = ql;
+        b.t = t;
+        b.toff = to;
+        b.tlen = tl;
+        int rc = phip_align(&b, res, tr);
+        free(q);
+        free(t);
+        if (rc != 0) {
+            snprintf(g_err, sizeof(g_err), "%s", phip_last_error());
+            fprintf(stderr, "%s: %s\n", __func__, g_err);
+            if (tr) { for (int k = 0; k < npairs; ++k) free(tr[k]); free(tr); }
+            free(ql); free(tl); free(qo); free(to); free(res);
+            return -1;
+        }
+        for (int k = 0; k < npairs; ++k) {
+            const phip_out_t *o = &res[k];
+            int w = width, sat = 0;
+            if (w == 0) {
+                w = (o->status & PHIP_ST_W32) ? 32 : 16;
+                if (alg == PHIP_SW && w == 32 && !phost_saturates(alg, 16, open, matrix, o)) w = 16;
+            } else {
+                sat = phost_saturates(alg, w, open, matrix, o);
+            }
+            int f = alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
+            if (alg == PHIP_SG) {
+                if (flags & PHIP_S1_BEG) f |= PARASAIL_FLAG_SG_S1_BEG;
+                if (flags & PHIP_S1_END) f |= PARASAIL_FLAG_SG_S1_END;
+                if (flags & PHIP_S2_BEG) f |= PARASAIL_FLAG_SG_S2_BEG;
+                if (flags & PHIP_S2_END) f |= PARASAIL_FLAG_SG_S2_END;
+            }
+            f |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
+            f |= w == 8 ? PARASAIL_FLAG_BITS_8 : w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32;
+            if (kind == PHIP_STATS) f |= PARASAIL_FLAG_STATS;
+            if (kind == PHIP_TRACE) f |= PARASAIL_FLAG_TRACE;
+            if (sat) f |= PARASAIL_FLAG_SATURATED;
+            if (out->score) out->score[k] = sat ? (alg == PHIP_SW ? (int)((1L << w) - 1) : 0) : o->score;
+            if (out->end_query) out->end_query[k] = sat ? 0 : o->end_query;
+            if (out->end_ref) out->end_ref[k] = sat ? 0 : o->end_ref;
+            if (out->matches) out->matches[k] = sat ? 0 : o->matches;
+            if (out->similar) out->similar[k] = sat ? 0 : o->similar;
+            if (out->length) out->length[k] = sat ? 0 : o->length;
+            if (out->flag) out->flag[k] = f;
+            if (tr) {
+                out->cigars[k] = sat ? NULL
+                    : phost_cigar_rowmajor(s1 + s1_off[k], ql[k], s2 + s2_off[k], tl[k],
+                                           alg == PHIP_SG, o->end_query, o->end_ref, tr[k]);
+                free(tr[k]);
+            }
+        }
+        free(tr);
+    }
+    free(ql); free(tl); free(qo); free(to); free(res);
+    return 0;
+oom:
+    free(ql); free(tl); free(qo); free(to); free(res);
+    fprintf(stderr, "%s: out of host memory\n", __func__);
+    return -1;
+}
+
+/* ---- resident batches ---- */
+
+struct parasail_hip_batch {
+    void *dev;
+    int npairs;
+    phip_matrix_t mx;
+    const parasail_matrix_t *matrix;
+    phip_out_t *res;
+};
+
+PHOST_EXPORT parasail_hip_batch_t *parasail_hip_batch_create(const char *s1, const int64_t *s1_off,
+                                                             const int *s1_len, const char *s2,
+                                                             const int64_t *s2_off,
+                                                             const int *s2_len, int npairs,
+                                                             const parasail_matrix_t *matrix)
+{
+    if (!matrix || !s1 || !s1_off || !s1_len || !s2 || !s2_off || !s2_len || npairs <= 0) {
+        fprintf(stderr, "%s: invalid arguments\n", __func__);
+        return NULL;
+    }
+    if (matrix->type == PARASAIL_MATRIX_TYPE_PSSM) {
+        fprintf(stderr, "%s: resident batches take a square matrix\n", __func__);
+        return NULL;
+    }
+    if (phip_init() != 0) {
+        errno = ENOSYS;
+        return NULL;
+    }
+    parasail_hip_batch_t *hb = (parasail_hip_batch_t *)calloc(1, sizeof(*hb));
+    int32_t *ql = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
+    int32_t *tl = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
+    int64_t *qo = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
+    int64_t *to = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
+    uint8_t *q = NULL, *t = NULL;
+    int64_t qtot = 0, ttot = 0;
+    if (!hb || !ql || !tl || !qo || !to) goto fail;
+    for (int k = 0; k < npairs; ++k) {
+        if (s1_len[k] <= 0 || s2_len[k] <= 0) {
+            fprintf(stderr, "%s: pair %d has an empty sequence\n", __func__, k);
+            goto fail;
+        }
+        ql[k] = s1_len[k];
+        tl[k] = s2_len[k];
+        qo[k] = qtot;
+        to[k] = ttot;
+        qtot += ql[k];
+        ttot += tl[k];
+    }
+    q = (uint8_t *)malloc((size_t)qtot);
+    t = (uint8_t *)malloc((size_t)ttot);
+    hb->res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)npairs);
+    if (!q || !t || !hb->res) goto fail;
+    for (int k = 0; k < npairs; ++k) {
+        const char *a = s1 + s1_off[k], *b = s2 + s2_off[k];
+        for (int i = 0; i < ql[k]; ++i) q[qo[k] + i] = (uint8_t)matrix->mapper[(unsigned char)a[i]];
+        for (int j = 0; j < tl[k]; ++j) t[to[k] + j] = (uint8_t)matrix->mapper[(unsigned char)b[j]];
+    }
+    {
+        phip_batch_t b;
+        memset(&b, 0, sizeof(b));
+        b.kind = PHIP_SCORE;
+        b.matrix.values = matrix->matrix;
+        b.matrix.size = matrix->size;
+        b.matrix.length = matrix->size;
+        b.matrix.max = matrix->max;
+        b.matrix.min = matrix->min;
+        b.npairs = npairs;
+        b.q = q;
+        b.qoff = qo;
+        b.qlen = ql;
+        b.t = t;
+        b.toff = to;
+        b.tlen = tl;
+        hb->dev = phip_resident_create(&b);
+    }
+    if (!hb->dev) {
+        fprintf(stderr, "%s: %s\n", __func__, phip_last_error());
+        goto fail;
+    }
+    hb->npairs = npairs;
+    hb->matrix = matrix;
+    free(q); free(t); free(ql); free(tl); free(qo); free(to);
+    return hb;
+fail:
+    if (hb) free(hb->res);
+    free(hb); free(q); free(t); free(ql); free(tl); free(qo); free(to);
+    return NULL;
+}
+
+PHOST_EXPORT int parasail_hip_batch_align(parasail_hip_batch_t *hb, const char *alg_name, int open,
+                                          int gap, parasail_hip_batch_result_t *out)
+{
+    int alg, flags;
+    if (!hb || !alg_name || parse_alg(alg_name, &alg, &flags) != 0 || open < 0 || gap < 0) {
+        fprintf(stderr, "%s: invalid arguments\n", __func__);
+        return -1;
+    }
+    if (phip_resident_align(hb->dev, alg, flags, open, gap, hb->res) != 0) {
+        fprintf(stderr, "%s: %s\n", __func__, phip_last_error());
+        return -1;
+    }
+    if (!out) return 0;
+    for (int k = 0; k < hb->npairs; ++k) {
+        const phip_out_t *o = &hb->res[k];
+        int w = (o->status & PHIP_ST_W32) ? 32 : 16;
+        if (alg == PHIP_SW && w == 32 && !phost_saturates(alg, 16, open, hb->matrix, o)) w = 16;
+        int f = alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
+        if (alg == PHIP_SG) {
+            if (flags & PHIP_S1_BEG) f |= PARASAIL_FLAG_SG_S1_BEG;
+            if (flags & PHIP_S1_END) f |= PARASAIL_FLAG_SG_S1_END;
+            if (flags & PHIP_S2_BEG) f |= PARASAIL_FLAG_SG_S2_BEG;
+            if (flags & PHIP_S2_END) f |= PARASAIL_FLAG_SG_S2_END;
+        }
+        f |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
+        f |= w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32;
+        if (out->score) out->score[k] = o->score;
+        if (out->end_query) out->end_query[k] = o->end_query;
+        if (out->end_ref) out->end_ref[k] = o->end_ref;
+        if (out->flag) out->flag[k] = f;
+    }
+    return 0;
+}
+
+PHOST_EXPORT void parasail_hip_batch_free(parasail_hip_batch_t *hb)
+{
+    if (!hb) return;
+    phip_resident_free(hb->dev);
+    free(hb->res);
+    free(hb);
+}

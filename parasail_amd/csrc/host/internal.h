@@ -1,0 +1,38 @@
+/* internal.h -- shared declarations of the C host layer. */
+#ifndef PHOST_INTERNAL_H
+#define PHOST_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../../include/parasail_hip.h"
+#include "../phip.h"
+
+#define PHOST_EXPORT __attribute__((visibility("default")))
+
+parasail_result_t *phost_result_new(void);
+parasail_result_t *phost_result_new_stats(void);
+parasail_result_t *phost_result_new_trace(int a, int b, size_t alignment, size_t size);
+parasail_profile_t *phost_profile_new(const char *s1, int s1Len, const parasail_matrix_t *matrix);
+void *phost_memalign(size_t alignment, size_t size);
+int phost_flag_lanes(int flag);
+int phost_flag_bits(int flag);
+int phost_trace_at(const void *t, int bits, long loc);
+
+/* one alignment through the device (align.c) */
+parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kind, int width,
+                               int scan, const char *s1, int s1Len, const char *s2, int s2Len,
+                               int open, int gap, const parasail_matrix_t *matrix,
+                               const parasail_profile_t *profile);
+
+/* saturation predicate of one width on exact DP values (align.c) */
+int phost_saturates(int alg, int width, int open, const parasail_matrix_t *matrix,
+                    const phip_out_t *o);
+
+/* CIGAR walk over a row-major int8 trace (cigar.c) */
+parasail_cigar_t *phost_cigar_rowmajor(const char *seqA, int lena, const char *seqB, int lenb,
+                                       int is_sg, int end_query, int end_ref, const int8_t *tr);
+
+extern const parasail_matrix_t phost_blosum62;
+
+#endif

@@ -1,0 +1,97 @@
+/*
+ * phip.h -- the thin C ABI between the C host layer (parasail_amd/csrc/host)
+ * and the HIP device layer (parasail_amd/csrc/device).  Plain C types only.
+ *
+ * One call aligns a batch of pairs that share an algorithm, result kind,
+ * gap penalties and a substitution matrix (or one PSSM).  The host layer
+ * builds every public parasail_* entry point on top of it.
+ */
+#ifndef PHIP_H
+#define PHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PHIP_SW = 0, PHIP_NW = 1, PHIP_SG = 2 };
+enum { PHIP_SCORE = 0, PHIP_STATS = 1, PHIP_TRACE = 2 };
+
+/* sg flag bits (parasail_sg_flags argument order, src/sg.c:31-35) */
+#define PHIP_S1_BEG 1
+#define PHIP_S1_END 2
+#define PHIP_S2_BEG 4
+#define PHIP_S2_END 8
+
+/* per-pair status bits returned in phip_out_t.status */
+#define PHIP_ST_SAT8 1   /* exact values leave the 8-bit window  */
+#define PHIP_ST_SAT16 2  /* exact values leave the 16-bit window */
+#define PHIP_ST_SAT32 4  /* exact values leave the 32-bit window */
+#define PHIP_ST_W32 8    /* computed by the int32 tier            */
+#define PHIP_ST_ERR 16   /* could not be computed                 */
+
+typedef struct phip_matrix {
+    const int *values;  /* row-major, `length` rows of `size` ints     */
+    int size;           /* columns (alphabet + 1)                      */
+    int length;         /* rows: size for square, query length for PSSM */
+    int is_pssm;
+    int max, min;
+} phip_matrix_t;
+
+typedef struct phip_batch {
+    int alg, kind, sg_flags;
+    int open, gap;
+    int want_sat_check;   /* compute PHIP_ST_SAT* bits (per-call widths) */
+    phip_matrix_t matrix;
+    int npairs;
+    /* mapped residue codes (matrix->mapper), host memory.  For a PSSM the
+     * query codes are still used by _stats (matches) and may be NULL
+     * otherwise; qlen must then equal matrix.length. */
+    const uint8_t *q;
+    const int64_t *qoff;
+    const int32_t *qlen;
+    const uint8_t *t;
+    const int64_t *toff;
+    const int32_t *tlen;
+} phip_batch_t;
+
+typedef struct phip_out {
+    int32_t score, end_query, end_ref;
+    int32_t matches, similar, length;
+    int32_t status;
+    int32_t min_val, max_val;  /* exact DP extrema (want_sat_check) */
+} phip_out_t;
+
+/* Initialise the device context once (thread-safe).  0 on success; on a
+ * host without a usable HIP device returns -1 and sets errno = ENOSYS. */
+int phip_init(void);
+const char *phip_last_error(void);
+
+/* Align every pair of the batch.  out[npairs]; for kind == PHIP_TRACE,
+ * traces[p] (if non-NULL) receives the row-major int8 trace codes of pair p
+ * (qlen*tlen bytes, the LOC_NOVEC layout of cigar.c:156).  0 on success. */
+int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **traces);
+
+/* Device-resident batch: residues, lengths and results stay in HBM across
+ * calls, and the wavefront plan for each scoring is built once.  Score-only
+ * alignments; NULL / -1 on failure. */
+void *phip_resident_create(const phip_batch_t *b);
+int phip_resident_align(void *handle, int alg, int sg_flags, int open, int gap, phip_out_t *out);
+void phip_resident_free(void *handle);
+
+/* banded global alignment (src/nw_banded.c) for one pair */
+int phip_nw_banded(const uint8_t *s1, int m, const uint8_t *s2, int n,
+                   int open, int gap, int k, const phip_matrix_t *mx,
+                   phip_out_t *out);
+
+int phip_device_info(int *cus, int *clock_khz);
+
+/* timing of the last phip_align call's dominant kernel (ms, HIP events) */
+double phip_last_kernel_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

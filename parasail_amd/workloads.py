@@ -1,0 +1,118 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+Protein residues are drawn iid over the 20 standard amino acids with the
+Robinson & Robinson background frequencies (Robinson A.B., Robinson L.R.,
+"Distribution of glutamine and asparagine residues and their near
+neighbors in peptides and proteins", PNAS 88:8880-8884, 1991).
+"""
+import numpy as np
+
+AA = "ARNDCQEGHILKMFPSTWYV"
+RR_PER_1000 = np.array([78.05, 51.29, 44.87, 53.64, 19.25, 42.64, 62.95, 73.77, 21.99, 51.42,
+                        90.19, 57.44, 22.43, 38.56, 52.03, 71.20, 58.41, 13.30, 32.16, 64.41])
+_AA_BYTES = np.frombuffer(AA.encode(), dtype=np.uint8)
+_DNA_BYTES = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+def lognormal_lengths(rng, n, median, sigma, lo, hi):
+    x = np.rint(median * np.exp(sigma * rng.standard_normal(n)))
+    return np.clip(x, lo, hi).astype(np.int64)
+
+
+def protein_bytes(rng, total):
+    p = RR_PER_1000 / RR_PER_1000.sum()
+    return _AA_BYTES[rng.choice(20, size=int(total), p=p)].tobytes()
+
+
+def concat_offsets(lengths):
+    off = np.zeros(len(lengths), dtype=np.int64)
+    if len(lengths):
+        off[1:] = np.cumsum(lengths)[:-1]
+    return off
+
+
+class PairBatch:
+    """Concatenated query and target residues with offsets and lengths."""
+
+    def __init__(self, qbytes, qlen, tbytes, tlen):
+        self.q = qbytes
+        self.qlen = np.asarray(qlen, dtype=np.int32)
+        self.qoff = concat_offsets(self.qlen)
+        self.t = tbytes
+        self.tlen = np.asarray(tlen, dtype=np.int32)
+        self.toff = concat_offsets(self.tlen)
+
+    def __len__(self):
+        return len(self.qlen)
+
+    @property
+    def cells(self):
+        return int(np.dot(self.qlen.astype(np.int64), self.tlen.astype(np.int64)))
+
+    def query(self, k):
+        o = int(self.qoff[k])
+        return self.q[o:o + int(self.qlen[k])]
+
+    def target(self, k):
+        o = int(self.toff[k])
+        return self.t[o:o + int(self.tlen[k])]
+
+    def subset(self, idx):
+        return PairBatch(b"".join(self.query(k) for k in idx), self.qlen[idx],
+                         b"".join(self.target(k) for k in idx), self.tlen[idx])
+
+
+def config0(seed=0, ntargets=2000):
+    """One 300-residue query against targets of length U[50, 1000]."""
+    rng = np.random.default_rng(seed)
+    q = protein_bytes(rng, 300)
+    tl = rng.integers(50, 1001, size=ntargets)
+    t = protein_bytes(rng, int(tl.sum()))
+    return q, PairBatch(q * ntargets, np.full(ntargets, 300), t, tl)
+
+
+def config1(seed=1, npairs=200000):
+    """Independent protein pairs, lengths log-normal (median 300, sigma 0.6,
+    clipped to [20, 5000])."""
+    rng = np.random.default_rng(seed)
+    ql = lognormal_lengths(rng, npairs, 300, 0.6, 20, 5000)
+    tl = lognormal_lengths(rng, npairs, 300, 0.6, 20, 5000)
+    return PairBatch(protein_bytes(rng, ql.sum()), ql, protein_bytes(rng, tl.sum()), tl)
+
+
+def _geom(rng, mean):
+    return int(rng.geometric(1.0 / mean))
+
+
+def mutate_dna(rng, s, sub, ins, dele, mean_indel):
+    """Substitutions at rate sub, insertions and deletions at the given
+    rates with geometric lengths of the given mean."""
+    out = bytearray()
+    i, n = 0, len(s)
+    while i < n:
+        r = rng.random()
+        if r < dele:
+            i += _geom(rng, mean_indel)
+            continue
+        if r < dele + ins:
+            out += _DNA_BYTES[rng.integers(0, 4, size=_geom(rng, mean_indel))].tobytes()
+        c = s[i]
+        if rng.random() < sub:
+            c = int(_DNA_BYTES[(int(np.nonzero(_DNA_BYTES == c)[0][0]) + rng.integers(1, 4)) % 4])
+        out.append(c)
+        i += 1
+    return bytes(out) if out else b"A"
+
+
+def config2(seed=2, npairs=100000):
+    """250 bp reads (4% substitutions, 0.5% insertions, 0.5% deletions,
+    geometric indel lengths of mean 2) against their 400 bp windows."""
+    rng = np.random.default_rng(seed)
+    win = _DNA_BYTES[rng.integers(0, 4, size=(npairs, 400))]
+    reads, rl = [], []
+    for k in range(npairs):
+        off = int(rng.integers(0, 151))
+        r = mutate_dna(rng, win[k, off:off + 250].tobytes(), 0.04, 0.005, 0.005, 2)
+        reads.append(r)
+        rl.append(len(r))
+    return PairBatch(b"".join(reads), rl, win.tobytes(), np.full(npairs, 400))

@@ -1,0 +1,93 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry
+point include/parasail_hip.h declares, resolves the reference's name grammar
+through its lookup tables, and refuses (NULL, no CPU fallback) when no
+device is usable."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import parasail_amd as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "parasail_hip.h")
+
+
+def declared_symbols():
+    pre = subprocess.run(["gcc", "-E", "-P", "-x", "c", HEADER], check=True,
+                         stdout=subprocess.PIPE, text=True).stdout
+    pre = re.sub(r"typedef[^;]*;", "", pre, flags=re.S)
+    names = set(re.findall(r"\b(parasail_\w+)\s*\(", pre))
+    return names
+
+
+def exported_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", P.LIB_PATH], check=True,
+                         stdout=subprocess.PIPE, text=True).stdout
+    return {l.split()[-1] for l in out.splitlines() if " T " in l}
+
+
+def test_every_declared_symbol_is_exported():
+    decl = declared_symbols()
+    assert len(decl) > 600  # 13 algs x 3 kinds x 2 strategies x 2 (profile) x 4 widths
+    missing = decl - exported_symbols()
+    assert not missing, sorted(missing)[:20]
+
+
+def test_lookup_follows_reference_grammar():
+    L = P.lib()
+    for alg in P.ALGS:
+        for kind in ("", "_stats", "_trace"):
+            for strat in ("striped", "scan"):
+                for w in ("8", "16", "32", "sat"):
+                    name = "parasail_%s%s_%s_hip_64_%s" % (alg, kind, strat, w)
+                    assert L.parasail_lookup_function(name.encode()), name
+                    # the reference also accepts the name without its prefix
+                    assert L.parasail_lookup_function(name[len("parasail_"):].encode())
+                    pname = "parasail_%s%s_%s_profile_hip_64_%s" % (alg, kind, strat, w)
+                    assert L.parasail_lookup_pfunction(pname.encode()), pname
+                    assert L.parasail_lookup_pcreator(pname.encode()), pname
+    assert not L.parasail_lookup_function(b"parasail_sw_striped_avx2_256_16")
+    assert not L.parasail_lookup_function(b"nonsense")
+    assert not L.parasail_lookup_function(None)
+
+
+def test_matrix_objects_cpu_side():
+    L = P.lib()
+    b = P.blosum62()
+    assert b.contents.size == 24 and b.contents.max == 11 and b.contents.min == -4
+    assert b.contents.mapper[ord("a")] == 0 and b.contents.mapper[ord("J")] == 23
+    assert not L.parasail_matrix_lookup(b"pam50")
+    m = P.matrix_create("ACGT", 2, -3)
+    mc = m.contents
+    assert mc.size == 5 and mc.alphabet == b"ACGT*" and mc.max == 2 and mc.min == -3
+    assert mc.matrix[0] == 2 and mc.matrix[1] == -3 and mc.matrix[4] == 0 and mc.matrix[20] == 0
+    assert mc.mapper[ord("t")] == 3 and mc.mapper[ord("N")] == 4
+    L.parasail_matrix_free(m)
+    p = L.parasail_matrix_convert_square_to_pssm(b, b"ARN", 3)
+    pc = p.contents
+    assert pc.type == 1 and pc.length == 3 and pc.size == 24
+    assert [pc.matrix[k] for k in (0, 24 + 1, 48 + 2)] == [4, 5, 6]
+    L.parasail_matrix_free(p)
+
+
+def test_cigar_codec_cpu_side():
+    L = P.lib()
+    L.parasail_cigar_encode_string.restype = ctypes.POINTER(P.Cigar)
+    L.parasail_cigar_encode_string.argtypes = [ctypes.c_char_p]
+    c = L.parasail_cigar_encode_string(b"3=2I2=1X4D14=")
+    s = ctypes.string_at(L.parasail_cigar_decode(c)).decode()
+    assert s == "3=2I2=1X4D14="
+    assert c.contents.seq[1] == (2 << 4) | 1
+    L.parasail_cigar_free(c)
+
+
+@pytest.mark.skipif(P.available(), reason="a HIP device is present")
+def test_no_device_returns_null():
+    """No CPU fallback: without a device every entry point returns NULL."""
+    r = P.align("sw_striped_hip_64_16", "ASDF", "ASDF", 10, 1, P.blosum62())
+    assert r is None
+    with pytest.raises(RuntimeError):
+        P.align_batch("sw", 0, 16, ["ASDF"], ["ASDF"], 10, 1, P.blosum62())

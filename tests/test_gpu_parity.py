@@ -1,0 +1,304 @@
+"""Parity of the hip family (through its C ABI) with the CPU oracle.
+
+Bit-exact on every pair: score, end_query, end_ref, matches/similar/length
+and CIGAR, for SW, NW and the eleven SG variants, per call and batched,
+square matrices and PSSMs, plus the reference's known answers
+(tests/test_basic.c:20-85).  The oracle is pinned in tests/test_oracle.py.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+import parasail_amd as P
+
+pytestmark = pytest.mark.gpu
+
+AA = "ARNDCQEGHILKMFPSTWYV"
+# Robinson & Robinson (1991) background frequencies, per 1000
+RR = [78.05, 51.29, 44.87, 53.64, 19.25, 42.64, 62.95, 73.77, 21.99, 51.42,
+      90.19, 57.44, 22.43, 38.56, 52.03, 71.20, 58.41, 13.30, 32.16, 64.41]
+
+
+def rseq(rng, n, alpha=AA, weights=None):
+    return "".join(rng.choices(alpha, weights=weights, k=n))
+
+
+def mutate(rng, s, alpha, sub=0.1, indel=0.03):
+    out = []
+    for c in s:
+        r = rng.random()
+        if r < sub:
+            out.append(rng.choice(alpha))
+        elif r < sub + indel / 2:
+            continue
+        elif r < sub + indel:
+            out.append(c)
+            out.append(rng.choice(alpha))
+        else:
+            out.append(c)
+    return "".join(out) or alpha[0]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_device():
+    if not P.available():
+        pytest.skip("no HIP device")
+
+
+@pytest.fixture(scope="module")
+def mats():
+    return {"blosum62": (P.blosum62(), O.blosum62()),
+            "dna": (P.matrix_create("ACGT", 2, -3), O.matrix_create("ACGT", 2, -3))}
+
+
+def oracle_alg(name):
+    if name == "sw":
+        return O.SW, (0, 0, 0, 0)
+    if name == "nw":
+        return O.NW, (0, 0, 0, 0)
+    return O.SG, P.SG_FLAGS[name]
+
+
+def check_one(name, kind, s1, s2, o, g, pm, om, width="sat"):
+    suffix = {"score": "", "stats": "_stats", "trace": "_trace"}[kind]
+    fn = "parasail_%s%s_striped_hip_64_%s" % (name, suffix, width)
+    got = P.align(fn, s1, s2, o, g, pm)
+    assert got is not None, fn
+    alg, flags = oracle_alg(name)
+    okind = {"score": O.SCORE, "stats": O.STATS, "trace": O.TRACE}[kind]
+    exp, tr = O.align(alg, okind, s1, s2, o, g, om, flags)
+    if got["flag"] & P.FLAG_SATURATED:
+        assert width != "sat"
+        return got
+    ctx = (fn, s1, s2, o, g)
+    assert (got["score"], got["end_query"], got["end_ref"]) == exp[:3], ctx
+    if kind == "stats":
+        assert (got["matches"], got["similar"], got["length"]) == exp[3:], ctx
+    if kind == "trace":
+        c = O.cigar(s1, s2, alg == O.SG, exp[1], exp[2], tr)
+        expc = None if c is None else (O.cigar_string(c[0]), c[1], c[2])
+        assert got.get("cigar") == expc, ctx
+    return got
+
+
+def test_known_answers(fixtures):
+    L = P.lib()
+    b = P.blosum62()
+    d = fixtures["matrices"]["pam50"]
+    pam = P.matrix_create(d["alphabet"][:-1], 0, 0)
+    L.parasail_matrix_set_value.argtypes = [P.MP, P.ctypes.c_int, P.ctypes.c_int, P.ctypes.c_int]
+    for r in range(24):
+        for c in range(24):
+            L.parasail_matrix_set_value(pam, r, c, d["values"][r * 24 + c])
+    acgt = P.matrix_create("acgt", 1, -1)
+    mats = {"blosum62": b, "pam50": pam}
+    for kat in fixtures["kat"]:
+        m = acgt if isinstance(kat["matrix"], list) else mats[kat["matrix"]]
+        for fn in ("sw_striped_hip_64_16", "sw_striped_hip_64_8", "sw_scan_hip_64_sat",
+                   "sw_striped_hip_64_32"):
+            r = P.align(fn, kat["s1"], kat["s2"], kat["open"], kat["gap"], m)
+            assert r["score"] == kat["score"], (kat["src"], fn)
+
+
+@pytest.mark.parametrize("name", P.ALGS)
+def test_per_call_all_algs(name, mats):
+    rng = random.Random(hash(name) & 0xffff)
+    for trial in range(12):
+        key = "blosum62" if trial % 2 == 0 else "dna"
+        pm, om = mats[key]
+        alpha = AA if key == "blosum62" else "ACGT"
+        s1 = rseq(rng, rng.randint(1, 180), alpha)
+        s2 = mutate(rng, s1, alpha) if trial % 3 == 0 else rseq(rng, rng.randint(1, 180), alpha)
+        o, g = rng.choice([(11, 1), (10, 1), (5, 2), (14, 2), (3, 1)])
+        for kind in ("score", "stats", "trace"):
+            check_one(name, kind, s1, s2, o, g, pm, om, width="sat")
+
+
+@pytest.mark.parametrize("width", ["8", "16", "32"])
+def test_fixed_widths_and_saturation(width, mats):
+    rng = random.Random(int(width))
+    pm, om = mats["blosum62"]
+    # a W-rich self alignment passes the 8-bit and (for length >= 3000) 16-bit windows
+    long_w = "".join(rng.choice("WWWWC") for _ in range(3100))
+    cases = [(rseq(rng, 50), rseq(rng, 60)), ("W" * 30, "W" * 30), (long_w, long_w)]
+    for s1, s2 in cases:
+        for name in ("sw", "nw", "sg"):
+            got = check_one(name, "score", s1, s2, 11, 1, pm, om, width=width)
+            sat = bool(got["flag"] & P.FLAG_SATURATED)
+            exp, _ = O.align(oracle_alg(name)[0], O.SCORE, s1, s2, 11, 1, om,
+                             oracle_alg(name)[1])
+            if name == "sw":
+                assert sat == (exp[0] > (1 << int(width)) - 11 - 2) if width != "32" else not sat
+                if sat:
+                    assert got["score"] == (1 << int(width)) - 1 and got["end_query"] == 0
+            elif sat:
+                assert got["score"] == 0 and got["end_ref"] == 0
+            bits = {"8": P.FLAG_BITS_8, "16": P.FLAG_BITS_16, "32": P.FLAG_BITS_32}[width]
+            assert got["flag"] & bits
+
+
+def test_sat_reports_narrowest_width(mats):
+    pm, om = mats["blosum62"]
+    r = P.align("parasail_sw_striped_hip_64_sat", "W" * 10, "W" * 10, 11, 1, pm)
+    assert r["score"] == 110 and r["flag"] & P.FLAG_BITS_8
+    r = P.align("parasail_sw_striped_hip_64_sat", "W" * 30, "W" * 30, 11, 1, pm)
+    assert r["score"] == 330 and r["flag"] & P.FLAG_BITS_16
+    assert not r["flag"] & P.FLAG_SATURATED
+
+
+def test_profile_functions(mats):
+    L = P.lib()
+    pm, om = mats["blosum62"]
+    rng = random.Random(4)
+    q = rseq(rng, 300)
+    L.parasail_profile_free.argtypes = [P.ctypes.c_void_p]
+    for w in ("16", "sat", "8"):
+        create = P.ctypes.CFUNCTYPE(P.ctypes.c_void_p, P.ctypes.c_char_p, P.ctypes.c_int, P.MP)(
+            L.parasail_lookup_pcreator(("parasail_sw_striped_profile_hip_64_" + w).encode()))
+        pf = P.ctypes.CFUNCTYPE(P.RP, P.ctypes.c_void_p, P.ctypes.c_char_p, P.ctypes.c_int,
+                                P.ctypes.c_int, P.ctypes.c_int)(
+            L.parasail_lookup_pfunction(("parasail_sw_striped_profile_hip_64_" + w).encode()))
+        qb = q.encode()  # the profile borrows s1 (src/memory.c:673): keep it alive
+        prof = create(qb, len(qb), pm)
+        assert prof
+        for _ in range(5):
+            t = rseq(rng, rng.randint(50, 1000))
+            r = pf(prof, t.encode(), len(t), 11, 1)
+            exp, _ = O.align(O.SW, O.SCORE, q, t, 11, 1, om)
+            rc = r.contents
+            if not rc.flag & P.FLAG_SATURATED:
+                assert (rc.score, rc.end_query, rc.end_ref) == exp[:3]
+            L.parasail_result_free(r)
+        L.parasail_profile_free(prof)
+
+
+def test_pssm(mats):
+    L = P.lib()
+    pm, om = mats["blosum62"]
+    rng = random.Random(8)
+    for _ in range(4):
+        s1 = rseq(rng, rng.randint(5, 120))
+        s2 = rseq(rng, rng.randint(5, 160))
+        pp = L.parasail_matrix_convert_square_to_pssm(pm, s1.encode(), len(s1))
+        op = O.convert_square_to_pssm(om, s1)
+        for name in ("sw", "nw", "sg", "sg_qb_de"):
+            alg, flags = oracle_alg(name)
+            exp, _ = O.align(alg, O.SCORE, None, s2, 11, 1, op, flags)
+            got = P.align("parasail_%s_striped_hip_64_16" % name, None, s2, 11, 1, pp)
+            assert (got["score"], got["end_query"], got["end_ref"]) == exp[:3]
+            # stats with a PSSM still use the consensus (sw_stats.c:62-64)
+            exps, _ = O.align(alg, O.STATS, s1, s2, 11, 1, op, flags)
+            gots = P.align("parasail_%s_stats_striped_hip_64_16" % name, s1, s2, 11, 1, pp)
+            assert (gots["score"], gots["matches"], gots["similar"], gots["length"]) == \
+                (exps[0],) + exps[3:]
+        L.parasail_matrix_free(pp)
+
+
+def test_edge_cases(mats):
+    pm, om = mats["blosum62"]
+    cases = [("A", "A"), ("A", "W"), ("W", "AAAAAAAAAA"), ("AAAAAAAAAA", "W"),
+             ("MKT", "MKT"), ("XXXJJZ", "BZX*J"), ("a" * 40, "A" * 39)]
+    for s1, s2 in cases:
+        for name in P.ALGS:
+            for kind in ("score", "stats", "trace"):
+                check_one(name, kind, s1, s2, 11, 1, pm, om)
+        check_one("sw", "score", s1, s2, 0, 0, pm, om)
+        check_one("nw", "score", s1, s2, 1, 0, pm, om)
+
+
+def test_invalid_arguments_return_null(mats):
+    pm, _ = mats["blosum62"]
+    f = P.function("parasail_sw_striped_hip_64_16")
+    assert not f(b"AC", 2, b"AC", 2, -1, 1, pm)
+    assert not f(b"AC", 2, b"AC", 0, 1, 1, pm)
+    assert not f(None, 2, b"AC", 2, 1, 1, pm)
+    assert not f(b"AC", 2, b"AC", 2, 1, 1, None)
+
+
+def lognormal_lengths(rng, n, median, sigma, lo, hi):
+    return np.clip(np.rint(median * np.exp(sigma * rng.standard_normal(n))), lo, hi).astype(int)
+
+
+@pytest.mark.parametrize("name", ["sw", "nw", "sg", "sg_qx", "sg_de", "sg_qb_db"])
+def test_batch_vs_oracle(name, mats):
+    """Config-1 shaped batch: log-normal lengths around 300, BLOSUM62 11/1."""
+    rng = np.random.default_rng(1)
+    prng = random.Random(2)
+    pm, om = mats["blosum62"]
+    n = 600
+    ql = lognormal_lengths(rng, n, 300, 0.6, 20, 5000)
+    tl = lognormal_lengths(rng, n, 300, 0.6, 20, 5000)
+    qs = [rseq(prng, int(k), AA, RR) for k in ql]
+    ts = [rseq(prng, int(k), AA, RR) for k in tl]
+    got = P.align_batch(name, 0, 0, qs, ts, 11, 1, pm)
+    alg, flags = oracle_alg(name)
+    exp, _ = O.batch(alg, O.SCORE, [om.map(q) for q in qs], [om.map(t) for t in ts],
+                     11, 1, om, flags)
+    bad = np.nonzero((got["score"] != exp[:, 0]) | (got["end_query"] != exp[:, 1]) |
+                     (got["end_ref"] != exp[:, 2]))[0]
+    assert bad.size == 0, [(int(k), int(ql[k]), int(tl[k]), got["score"][k], exp[k, :3].tolist())
+                           for k in bad[:5]]
+
+
+def test_batch_long_and_ragged(mats):
+    """Multi-band queries (> 512 rows), ragged target lengths, short pairs."""
+    prng = random.Random(3)
+    pm, om = mats["blosum62"]
+    qs, ts = [], []
+    for k in range(60):
+        m = prng.choice([1, 2, 7, 63, 64, 65, 511, 512, 513, 900, 1500, 2600])
+        q = rseq(prng, m)
+        t = mutate(prng, q, AA) if k % 2 else rseq(prng, prng.randint(1, 1400))
+        qs.append(q)
+        ts.append(t)
+    for name in ("sw", "nw", "sg"):
+        got = P.align_batch(name, 0, 0, qs, ts, 11, 1, pm)
+        alg, flags = oracle_alg(name)
+        exp, _ = O.batch(alg, O.SCORE, [om.map(q) for q in qs], [om.map(t) for t in ts],
+                         11, 1, om, flags)
+        for k in range(len(qs)):
+            assert (got["score"][k], got["end_query"][k], got["end_ref"][k]) == \
+                tuple(exp[k, :3]), (name, k, len(qs[k]), len(ts[k]))
+
+
+def test_batch_stats_trace_dna(mats):
+    """Config-2 shaped: 250 bp reads in 400 bp windows, ACGT 2/-3, 5/2."""
+    prng = random.Random(5)
+    pm, om = mats["dna"]
+    qs, ts = [], []
+    for _ in range(200):
+        w = rseq(prng, 400, "ACGT")
+        off = prng.randint(0, 150)
+        qs.append(mutate(prng, w[off:off + 250], "ACGT", sub=0.04, indel=0.01))
+        ts.append(w)
+    for name in ("sg", "sg_qx", "sw", "nw"):
+        alg, flags = oracle_alg(name)
+        st = P.align_batch(name, 1, 16, qs, ts, 5, 2, pm)
+        exp, _ = O.batch(alg, O.STATS, [om.map(q) for q in qs], [om.map(t) for t in ts],
+                         5, 2, om, flags)
+        for col, key in enumerate(("score", "end_query", "end_ref", "matches", "similar", "length")):
+            assert (st[key] == exp[:, col]).all(), (name, key)
+        tr = P.align_batch(name, 2, 16, qs, ts, 5, 2, pm, want_cigar=True)
+        for k in range(0, len(qs), 7):
+            e, t = O.align(alg, O.TRACE, qs[k], ts[k], 5, 2, om, flags)
+            c = O.cigar(qs[k], ts[k], alg == O.SG, e[1], e[2], t)
+            expc = None if c is None else (O.cigar_string(c[0]), c[1], c[2])
+            assert tr["cigar"][k] == expc, (name, k)
+
+
+def test_banded(mats):
+    L = P.lib()
+    pm, om = mats["dna"]
+    prng = random.Random(6)
+    for _ in range(10):
+        s1 = rseq(prng, prng.randint(1, 400), "ACGT")
+        s2 = mutate(prng, s1, "ACGT", sub=0.1, indel=0.04)
+        for k in (1, 3, 16, 256):
+            r = L.parasail_nw_banded_hip(s1.encode(), len(s1), s2.encode(), len(s2), 5, 2, k, pm)
+            exp = O.nw_banded(s1, s2, 5, 2, k, om)
+            rc = r.contents
+            assert (rc.score, rc.end_query, rc.end_ref) == exp[:3]
+            assert rc.flag & P.FLAG_BANDED and rc.flag & P.FLAG_NW
+            L.parasail_result_free(r)
