@@ -86,8 +86,11 @@ struct WaveCfg {
     const int2 *wdims;        // per wave: (max bands, max columns)
     int nwaves;
     const uint32_t *colcode;  // per job column stream: (toffA | toffB << 16)
-    const short *tab;         // (size+2) x TS table of W+open
-    int tabn, TS, VIRT, PADQ, PADT;
+    const short *tab;         // bank-replicated table of W+open (see k_wave16)
+    int tabn;                 // table size in dwords
+    int qstride, ncopy;       // bytes per query-code row; copies (32 or 1)
+    int VIRT, PADQ, PADT;
+    int *next_wave;           // work-queue counter
     const uint8_t *q;
     int2 *dump;               // per-thread slot for stores a lane does not need
     int2 *bandbuf;
@@ -95,222 +98,234 @@ struct WaveCfg {
     int open, gap, alg, sg_flags;
 };
 
+// Score table in LDS: for query code q and target code t the int16
+// W(q,t)+open sits at byte  q*qstride + (t>>1)*128*(ncopy/32) + (t&1)*2 + c*4
+// with c = lane % ncopy: with 32 copies every lane of a half-wave reads its
+// own bank, so the random (q, t) lookups of a wave never conflict.
 template <int R, int T, bool SW>
-__global__ void __launch_bounds__(256) k_wave16(WaveCfg c)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R <= 12 ? 3 : 2))) k_wave16(WaveCfg c)
 {
-    extern __shared__ short tab[];
-    for (int k = threadIdx.x; k < c.tabn; k += blockDim.x) tab[k] = c.tab[k];
+    extern __shared__ int tab32[];
+    for (int k = threadIdx.x; k < c.tabn; k += blockDim.x) tab32[k] = ((const int *)c.tab)[k];
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
-    const int wv = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (wv >= c.nwaves) return;
     const int t = lane % T;
-    const int jobi = wv * (64 / T) + lane / T;
-    const bool live = jobi < c.njobs;
-    WaveJob J;
-    if (live) {
-        J = c.jobs[jobi];
-    } else {
-        memset(&J, 0, sizeof(J));
-        J.pair[0] = J.pair[1] = -1;
-    }
-    const int2 wd = c.wdims[wv];
     const int open = c.open, gap = c.gap;
     const int openv = pack16(open, open), gapv = pack16(gap, gap);
     const bool s1_beg = c.alg == PHIP_SG && (c.sg_flags & PHIP_S1_BEG);
     const bool s2_beg = c.alg == PHIP_SG && (c.sg_flags & PHIP_S2_BEG);
     const bool zero_top = SW || s2_beg;     // row 0 of the DP is all zeros
     const bool zero_left = SW || s1_beg;    // column 0 of the DP is all zeros
-    const int rowbytes = c.TS * 2;
-    const char *tabb = (const char *)tab;
+    const char *tabb = (const char *)tab32 + (lane % c.ncopy) * 4;
+    int2 *const dump = c.dump + (threadIdx.x + blockIdx.x * blockDim.x);
 
-    // SW: job-level candidate per half
-    int cS[2] = {-1, -1}, cC[2] = {0, 0}, cR[2] = {0, 0};
-    // NW/SG: last-column maximum (rows 0..m-2) and last-row maximum per half
-    int cmx[2] = {PHIP_NEG_INF32, PHIP_NEG_INF32}, crow[2] = {0, 0};
-    int rmx[2] = {PHIP_NEG_INF32, PHIP_NEG_INF32}, rj[2] = {0, 0}, corner[2] = {0, 0};
+    // persistent: each wave takes the next group of 64/T jobs off the queue
+    // (jobs are ordered longest first, so the tail is short)
+    for (;;) {
+        int wv = 0;
+        if (lane == 0) wv = atomicAdd(c.next_wave, 1);
+        wv = __builtin_amdgcn_readfirstlane(wv);
+        if (wv >= c.nwaves) break;
+        const int jobi = wv * (64 / T) + lane / T;
+        const bool live = jobi < c.njobs;
+        WaveJob J;
+        if (live) {
+            J = c.jobs[jobi];
+        } else {
+            memset(&J, 0, sizeof(J));
+            J.pair[0] = J.pair[1] = -1;
+        }
+        const int2 wd = c.wdims[wv];
 
-    for (int b = 0; b < wd.x; ++b) {
-        const bool band_live = live && b < J.nb;
-        const bool last_band = b == J.nb - 1;
-        int qo[2][R];
-        int HL[R], E[R];
-        int first_row[2];
+        // SW: job-level candidate per half
+        int cS[2] = {-1, -1}, cC[2] = {0, 0}, cR[2] = {0, 0};
+        // NW/SG: last-column maximum (rows 0..m-2) and last-row maximum per half
+        int cmx[2] = {PHIP_NEG_INF32, PHIP_NEG_INF32}, crow[2] = {0, 0};
+        int rmx[2] = {PHIP_NEG_INF32, PHIP_NEG_INF32}, rj[2] = {0, 0}, corner[2] = {0, 0};
+
+        for (int b = 0; b < wd.x; ++b) {
+            const bool band_live = live && b < J.nb;
+            const bool last_band = b == J.nb - 1;
+            int qo[R];  // table row offsets: half A in bits 0-15, half B in 16-31
+            int HL[R], E[R];
+            int first_row[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            first_row[h] = b * T * R + t * R - J.s[h];
+            for (int h = 0; h < 2; ++h) first_row[h] = b * T * R + t * R - J.s[h];
 #pragma unroll
             for (int i = 0; i < R; ++i) {
-                int g = first_row[h] + i;
-                int code;
-                if (J.pair[h] < 0) code = c.PADQ;
-                else if (g < 0) code = c.VIRT;
-                else if (g < J.m[h]) code = c.q[J.qoff[h] + g];
-                else code = c.PADQ;
-                qo[h][i] = code * rowbytes;
-            }
-        }
-        // left boundary (DP column 0) as H-open, per row and half
-        auto left = [&](int g) -> int {
-            if (g < 0 || zero_left) return -open;          // virtual rows copy H[0][0] = 0
-            return clamp16(-open - g * gap - open);         // H[g+1][0] = -open - g*gap
-        };
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            HL[i] = pack16(left(first_row[0] + i), left(first_row[1] + i));
-            E[i] = PK_NEG;
-        }
-        int prevRH = pack16(left(first_row[0] - 1), left(first_row[1] - 1));
-        if (b == 0 && t == 0) prevRH = pack16(-open, -open);  // H[0][0] = 0
-        int botH = PK_NEG, botF = PK_NEG;
-        int best = (int)0xFFFFFFFF, bcol = 0;
-        int snap[R];
-#pragma unroll
-        for (int i = 0; i < R; ++i) snap[i] = 0;
-
-        const int steps = wd.y + T - 1;
-        const int64_t bbstride = (J.ncols + 15) & ~15;
-        const int2 *bbin = c.bandbuf + J.bboff + (int64_t)(b - 1) * bbstride;
-        int2 *bbout = c.bandbuf + J.bboff + (int64_t)b * bbstride;
-        const uint32_t *cc = c.colcode + J.ccoff;
-        // column codes and band-boundary rows are fetched two steps ahead so
-        // the HBM latency overlaps the two steps' arithmetic
-        // loads are unconditional (clamped index, valid pointer) so that the
-        // wait-count for the value fetched two steps earlier stays precise
-        // (the stream holds pad codes up to the wave's widest job, so a
-        // clamped index is always a valid code; band-boundary values past
-        // the job's columns only feed columns whose results are unused)
-        const int2 *bbsrc = (b > 0 && band_live) ? bbin : c.bandbuf;
-        const int lastcc = J.cclen > 0 ? J.cclen - 1 : 0;
-        const int lastc = (b > 0 && band_live && J.ncols > 0) ? J.ncols - 1 : 0;
-        auto ld_code = [&](int jj) -> uint32_t {
-            return cc[jj < 0 ? 0 : (jj > lastcc ? lastcc : jj)];
-        };
-        auto ld_bb = [&](int jj) -> int2 {
-            return bbsrc[jj < 0 ? 0 : (jj > lastc ? lastc : jj)];
-        };
-        uint32_t code1 = ld_code(-t), code2 = ld_code(1 - t);
-        int2 bb1 = ld_bb(-t), bb2 = ld_bb(1 - t);
-        for (int s = 0; s < steps; ++s) {
-            const int j = s - t;
-            // exchange with lane t-1 while every lane is enabled
-            const int xH = shr1(botH, botH);
-            const int xF = shr1(botF, botF);
-            const uint32_t code = code1;
-            const int2 bb = bb1;
-            code1 = code2;
-            bb1 = bb2;
-            code2 = ld_code(j + 2);
-            bb2 = ld_bb(j + 2);
-            if (j < 0 || j >= wd.y) continue;  // lane outside the band's columns
-            const bool in = band_live && j < J.ncols;
-            int rH = xH, rF = xF;
-            if (t == 0) {
-                // group head: DP row 0 (band 0) or the previous band's bottom row
-                if (b == 0) {
-                    int v = zero_top ? -open : clamp16(-open - j * gap - open);
-                    rH = pack16(v, v);
-                    rF = PK_NEG;
-                } else {
-                    rH = bb.x;
-                    rF = bb.y;
-                }
-            }
-            int diag = prevRH;
-            prevRH = rH;
-            int up = rH, F = rF;
-            const int toA = code & 0xffff, toB = code >> 16;
-            int cm = 0;
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-                const short wa = *(const short *)(tabb + qo[0][i] + toA);
-                const short wb = *(const short *)(tabb + qo[1][i] + toB);
-                const int W = pack16(wa, wb);
-                const int t1 = pk_add(diag, W);
-                E[i] = pk_max(pk_sub(E[i], gapv), HL[i]);
-                F = pk_max(pk_sub(F, gapv), up);
-                int h = pk_max(pk_max(t1, E[i]), F);
-                if (SW) h = pk_max(h, 0);
-                diag = HL[i];
-                HL[i] = pk_sub(h, openv);
-                up = HL[i];
-                if (SW) cm = i == 0 ? h : pk_max(cm, h);
-            }
-            botH = HL[R - 1];
-            botF = F;
-            {
-                // unconditional store (inactive lanes write their own dump
-                // slot) keeps the loop free of memory ops under branches
-                const bool wr = !last_band && t == T - 1 && in;
-                int2 *dst = wr ? bbout + j : c.dump + (threadIdx.x + blockIdx.x * blockDim.x);
-                *dst = make_int2(botH, botF);
-            }
-            if (SW) {
-                const int nbst = pk_max(best, cm);
-                if (nbst != best) {
-                    const int d = nbst ^ best;
-                    const int mask = ((d & 0xffff) ? 0xffff : 0) | ((d >> 16) ? (int)0xffff0000 : 0);
-#pragma unroll
-                    for (int i = 0; i < R; ++i) snap[i] = (HL[i] & mask) | (snap[i] & ~mask);
-                    bcol = (pack16(j, j) & mask) | (bcol & ~mask);
-                    best = nbst;
-                }
-            } else if (band_live) {
-                // the last query row is the bottom row of lane T-1 in the
-                // last band (bottom alignment): its running maximum (first
-                // column on ties) and its value at the last column
-                if (last_band && t == T - 1) {
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int v = (h ? hi16(botH) : lo16(botH)) + open;
-                        if (j < J.n[h] && v > rmx[h]) { rmx[h] = v; rj[h] = j; }
-                        if (j == J.n[h] - 1) corner[h] = v;
-                    }
-                }
-                // the last column: largest value over rows 0..m-2, first row
+                int off[2];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    if (j == J.n[h] - 1) {
+                    const int g = first_row[h] + i;
+                    int code;
+                    if (J.pair[h] < 0) code = c.PADQ;
+                    else if (g < 0) code = c.VIRT;
+                    else if (g < J.m[h]) code = c.q[J.qoff[h] + g];
+                    else code = c.PADQ;
+                    off[h] = code * c.qstride;
+                }
+                qo[i] = off[0] | (off[1] << 16);
+            }
+            // left boundary (DP column 0) as H-open, per row and half
+            auto left = [&](int g) -> int {
+                if (g < 0 || zero_left) return -open;      // virtual rows copy H[0][0] = 0
+                return clamp16(-open - g * gap - open);     // H[g+1][0] = -open - g*gap
+            };
 #pragma unroll
-                        for (int i = 0; i < R; ++i) {
-                            const int g = first_row[h] + i;
-                            const int v = (h ? hi16(HL[i]) : lo16(HL[i])) + open;
-                            if (g >= 0 && g < J.m[h] - 1 && v > cmx[h]) { cmx[h] = v; crow[h] = g; }
+            for (int i = 0; i < R; ++i) {
+                HL[i] = pack16(left(first_row[0] + i), left(first_row[1] + i));
+                E[i] = PK_NEG;
+            }
+            int prevRH = pack16(left(first_row[0] - 1), left(first_row[1] - 1));
+            if (b == 0 && t == 0) prevRH = pack16(-open, -open);  // H[0][0] = 0
+            int botH = PK_NEG, botF = PK_NEG;
+            int best = (int)0xFFFFFFFF, bcol = 0;
+            int snap[SW ? R : 1];
+#pragma unroll
+            for (int i = 0; i < (SW ? R : 1); ++i) snap[i] = 0;
+
+            const int steps = wd.y + T - 1;
+            const int64_t bbstride = (J.ncols + 15) & ~15;
+            const int2 *bbin = c.bandbuf + J.bboff + (int64_t)(b - 1) * bbstride;
+            int2 *bbout = c.bandbuf + J.bboff + (int64_t)b * bbstride;
+            const uint32_t *cc = c.colcode + J.ccoff;
+            // column codes and band-boundary rows are fetched two steps ahead
+            // with unconditional loads (the stream holds pad codes up to the
+            // wave's widest job, band-boundary values past the job's columns
+            // only feed columns whose results are unused), so the wait for
+            // the value fetched earlier stays precise
+            const int2 *bbsrc = (b > 0 && band_live) ? bbin : c.bandbuf;
+            const int lastcc = J.cclen > 0 ? J.cclen - 1 : 0;
+            const int lastc = (b > 0 && band_live && J.ncols > 0) ? J.ncols - 1 : 0;
+            auto ld_code = [&](int jj) -> uint32_t {
+                return cc[jj < 0 ? 0 : (jj > lastcc ? lastcc : jj)];
+            };
+            auto ld_bb = [&](int jj) -> int2 {
+                return bbsrc[jj < 0 ? 0 : (jj > lastc ? lastc : jj)];
+            };
+            uint32_t code1 = ld_code(-t), code2 = ld_code(1 - t);
+            int2 bb1 = ld_bb(-t), bb2 = ld_bb(1 - t);
+            for (int s = 0; s < steps; ++s) {
+                const int j = s - t;
+                // exchange with lane t-1 while every lane is enabled
+                const int xH = shr1(botH, botH);
+                const int xF = shr1(botF, botF);
+                const uint32_t code = code1;
+                const int2 bb = bb1;
+                code1 = code2;
+                bb1 = bb2;
+                code2 = ld_code(j + 2);
+                bb2 = ld_bb(j + 2);
+                if (j < 0 || j >= wd.y) continue;  // lane outside the band's columns
+                const bool in = band_live && j < J.ncols;
+                int rH = xH, rF = xF;
+                if (t == 0) {
+                    // group head: DP row 0 (band 0) or the previous band's bottom row
+                    if (b == 0) {
+                        const int v = zero_top ? -open : clamp16(-open - j * gap - open);
+                        rH = pack16(v, v);
+                        rF = PK_NEG;
+                    } else {
+                        rH = bb.x;
+                        rF = bb.y;
+                    }
+                }
+                int diag = prevRH;
+                prevRH = rH;
+                int up = rH, F = rF;
+                const int toA = code & 0xffff, toB = code >> 16;
+                int cm = 0;
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const short wa = *(const short *)(tabb + (qo[i] & 0xffff) + toA);
+                    const short wb = *(const short *)(tabb + ((unsigned)qo[i] >> 16) + toB);
+                    const int W = pack16(wa, wb);
+                    const int t1 = pk_add(diag, W);
+                    E[i] = pk_max(pk_sub(E[i], gapv), HL[i]);
+                    F = pk_max(pk_sub(F, gapv), up);
+                    int h = pk_max(pk_max(t1, E[i]), F);
+                    if (SW) h = pk_max(h, 0);
+                    diag = HL[i];
+                    HL[i] = pk_sub(h, openv);
+                    up = HL[i];
+                    if (SW) cm = i == 0 ? h : pk_max(cm, h);
+                }
+                botH = HL[R - 1];
+                botF = F;
+                {
+                    // unconditional store (inactive lanes write their own dump
+                    // slot) keeps the loop free of memory ops under branches
+                    const bool wr = !last_band && t == T - 1 && in;
+                    int2 *dst = wr ? bbout + j : dump;
+                    *dst = make_int2(botH, botF);
+                }
+                if (SW) {
+                    const int nbst = pk_max(best, cm);
+                    if (nbst != best) {
+                        const int d = nbst ^ best;
+                        const int mask = ((d & 0xffff) ? 0xffff : 0) | ((d >> 16) ? (int)0xffff0000 : 0);
+#pragma unroll
+                        for (int i = 0; i < (SW ? R : 1); ++i) snap[i] = (HL[i] & mask) | (snap[i] & ~mask);
+                        bcol = (pack16(j, j) & mask) | (bcol & ~mask);
+                        best = nbst;
+                    }
+                } else if (band_live) {
+                    // the last query row is the bottom row of lane T-1 in the
+                    // last band (bottom alignment): its running maximum (first
+                    // column on ties) and its value at the last column
+                    if (last_band && t == T - 1) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int v = (h ? hi16(botH) : lo16(botH)) + open;
+                            if (j < J.n[h] && v > rmx[h]) { rmx[h] = v; rj[h] = j; }
+                            if (j == J.n[h] - 1) corner[h] = v;
+                        }
+                    }
+                    // the last column: largest value over rows 0..m-2, first row
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (j == J.n[h] - 1) {
+#pragma unroll
+                            for (int i = 0; i < R; ++i) {
+                                const int g = first_row[h] + i;
+                                const int v = (h ? hi16(HL[i]) : lo16(HL[i])) + open;
+                                if (g >= 0 && g < J.m[h] - 1 && v > cmx[h]) { cmx[h] = v; crow[h] = g; }
+                            }
                         }
                     }
                 }
             }
+            if (SW && band_live) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (J.pair[h] < 0) continue;
+                    const int S = h ? hi16(best) : lo16(best);
+                    const int col = h ? hi16(bcol) : lo16(bcol);
+                    int row = -1;
+#pragma unroll
+                    for (int i = (SW ? R : 1) - 1; i >= 0; --i) {
+                        const int g = first_row[h] + i;
+                        const int v = (h ? hi16(snap[i]) : lo16(snap[i])) + open;
+                        if (g >= 0 && g < J.m[h] && v == S) row = g;
+                    }
+                    if (row >= 0 && (S > cS[h] || (S == cS[h] && col < cC[h]))) {
+                        cS[h] = S; cC[h] = col; cR[h] = row;
+                    }
+                }
+            }
+            __threadfence();
         }
-        if (SW && band_live) {
+        if (live) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 if (J.pair[h] < 0) continue;
-                const int S = h ? hi16(best) : lo16(best);
-                const int col = h ? hi16(bcol) : lo16(bcol);
-                int row = -1;
-#pragma unroll
-                for (int i = R - 1; i >= 0; --i) {
-                    int g = first_row[h] + i;
-                    int v = (h ? hi16(snap[i]) : lo16(snap[i])) + open;
-                    if (g >= 0 && g < J.m[h] && v == S) row = g;
+                if (SW) {
+                    c.cand[J.cdoff[h] + t] = make_int4(cS[h], cC[h], cR[h], 0);
+                } else {
+                    c.cand[J.cdoff[h] + 2 * t] = make_int4(cmx[h], crow[h], rmx[h], rj[h]);
+                    c.cand[J.cdoff[h] + 2 * t + 1] = make_int4(corner[h], 0, 0, 0);
                 }
-                if (row >= 0 && (S > cS[h] || (S == cS[h] && col < cC[h]))) {
-                    cS[h] = S; cC[h] = col; cR[h] = row;
-                }
-            }
-        }
-        __threadfence();
-    }
-    if (live) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            if (J.pair[h] < 0) continue;
-            if (SW) {
-                c.cand[J.cdoff[h] + t] = make_int4(cS[h], cC[h], cR[h], 0);
-            } else {
-                c.cand[J.cdoff[h] + 2 * t] = make_int4(cmx[h], crow[h], rmx[h], rj[h]);
-                c.cand[J.cdoff[h] + 2 * t + 1] = make_int4(corner[h], 0, 0, 0);
             }
         }
     }
@@ -390,7 +405,7 @@ __global__ void k_sg_reduce(const int4 *cand, const int64_t *cdoff, const int32_
 // Column-code stream: per job and column, the byte offsets of the two
 // target residues' columns in the LDS table.
 __global__ void k_colcode(const WaveJob *jobs, int njobs, const uint8_t *t, const int64_t *toff,
-                          int PADT, uint32_t *colcode)
+                          int PADT, int ncopy, uint32_t *colcode)
 {
     const int jb = blockIdx.x;
     if (jb >= njobs) return;
@@ -400,7 +415,9 @@ __global__ void k_colcode(const WaveJob *jobs, int njobs, const uint8_t *t, cons
     for (int j = threadIdx.x; j < J.cclen; j += blockDim.x) {
         uint32_t a = (ta && j < J.n[0]) ? ta[j] : PADT;
         uint32_t bb = (tb && j < J.n[1]) ? tb[j] : PADT;
-        colcode[J.ccoff + j] = (a * 2) | ((bb * 2) << 16);
+        const uint32_t oa = (a >> 1) * ncopy * 4 + (a & 1) * 2;
+        const uint32_t ob = (bb >> 1) * ncopy * 4 + (bb & 1) * 2;
+        colcode[J.ccoff + j] = oa | (ob << 16);
     }
 }
 
@@ -425,7 +442,8 @@ Shape choose(int m, int n)
     for (int nb = 1; nb <= maxnb && nb <= 256; ++nb) {
         for (int R : kRs) {
             if ((int64_t)nb * kT * R < m) continue;
-            double cost = (double)nb * kT * R * (n + kT - 1) + nb * 200.0;
+            // VALU work: ~13 ops per row and ~40 per step of each lane
+            double cost = (double)nb * (n + kT - 1) * (13.0 * R + 40.0);
             if (cost < bestc) { bestc = cost; best = Shape{R, nb}; }
             break;
         }
@@ -480,6 +498,7 @@ struct WavePlan {
     std::vector<Launch> launches;
     int nfast = 0, nslow = 0;
     int tabrows = 0, TS = 0, VIRT = 0, PADQ = 0, PADT = 0, ovf_limit = 0;
+    int tabn = 0, qstride = 0, ncopy = 1;
     int maxn = 0, simple_lanes = 0;
     char *base = nullptr;
     WaveJob *djobs = nullptr;
@@ -527,10 +546,12 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
             slow.push_back(i);
         }
     }
+    // per R (one kernel instance each), longest work first so that the
+    // queue ends on short jobs
     std::sort(fast.begin(), fast.end(), [&](int x, int y) {
         if (shape[x].R != shape[y].R) return shape[x].R < shape[y].R;
-        if (shape[x].nb != shape[y].nb) return shape[x].nb < shape[y].nb;
-        return hb.tlen[x] < hb.tlen[y];
+        if (shape[x].nb != shape[y].nb) return shape[x].nb > shape[y].nb;
+        return hb.tlen[x] > hb.tlen[y];
     });
     // LDS table: rows [0,size) real, VIRT, PADQ; columns [0,size) real, PADT
     const int size = mx.size;
@@ -539,16 +560,25 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->PADQ = size + 1;
     P->PADT = size;
     P->tabrows = size + 2;
-    std::vector<short> htab((size_t)P->tabrows * P->TS, (short)W_NEG);
+    // bank-replicated layout (32 copies) when it fits comfortably in LDS
+    const int tsp = (P->TS + 1) & ~1;
+    P->ncopy = ((size_t)P->tabrows * (tsp / 2) * 32 * 4 <= (48u << 10)) ? 32 : 1;
+    P->qstride = (tsp / 2) * P->ncopy * 4;
+    P->tabn = P->tabrows * (tsp / 2) * P->ncopy;
+    std::vector<short> htab((size_t)P->tabn * 2, (short)W_NEG);
     const bool s2_beg = dp.alg == PHIP_SG && (dp.sg_flags & PHIP_S2_BEG);
     if (!mx.is_pssm) {
         for (int r = 0; r < P->tabrows; ++r)
-            for (int col = 0; col < P->TS; ++col) {
+            for (int col = 0; col < tsp; ++col) {
                 int v = W_NEG;
-                if (col == P->PADT || r == P->PADQ) v = W_NEG;
+                if (col >= P->PADT || r == P->PADQ) v = W_NEG;
                 else if (r == P->VIRT) v = (sw || s2_beg) ? dp.open : W_NEG;
                 else v = mx.values[(size_t)r * size + col] + dp.open;
-                htab[(size_t)r * P->TS + col] = (short)v;
+                for (int cp = 0; cp < P->ncopy; ++cp) {
+                    size_t byte = (size_t)r * P->qstride + (size_t)(col >> 1) * P->ncopy * 4 +
+                                  (col & 1) * 2 + cp * 4;
+                    htab[byte / 2] = (short)v;
+                }
             }
     }
     P->ovf_limit = 32767 - (mx.max + dp.open + 1);
@@ -636,7 +666,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
     size_t o_jobs = slot(nj * sizeof(WaveJob)), o_tab = slot(htab.size() * 2),
            o_cc = slot((size_t)ccoff * 4), o_bb = slot((size_t)bboff * 8),
-           o_dump = slot(((wdims.size() + 3) / 4 + 1) * 256 * 8), o_cd = slot((size_t)cdoff * 16),
+           o_dump = slot((size_t)1024 * 256 * 8), o_cd = slot((size_t)cdoff * 16),
            o_pairs = slot(rpairs.size() * 4), o_m = slot(rm.size() * 4), o_n = slot(rn.size() * 4),
            o_cdo = slot(rcd.size() * 8),
            o_wd = slot(wdims.size() * 8), o_slow = slot(slow.size() * 4),
@@ -678,7 +708,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     rc |= up(P->dslow, slow.data(), slow.size() * 4);
     if (nj) {
         hipLaunchKernelGGL(k_colcode, dim3((unsigned)nj), dim3(256), 0, stream, P->djobs, (int)nj,
-                           dp.t, dp.toff, P->PADT, P->dcc);
+                           dp.t, dp.toff, P->PADT, P->ncopy, P->dcc);
         if (hipGetLastError() != hipSuccess) rc = -1;
     }
     // the pageable host vectors die here: wait for the copies
@@ -691,9 +721,37 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     return P;
 }
 
+namespace {
+
+// Side streams on which the per-R kernels of one pass run concurrently, so
+// one kernel's tail overlaps the next one's start.
+constexpr int kSide = 4;
+hipStream_t g_side[kSide];
+hipEvent_t g_fork, g_join[kSide];
+bool g_side_ok = false;
+
+int side_streams()
+{
+    if (g_side_ok) return 0;
+    for (int k = 0; k < kSide; ++k) {
+        HIP_OK(hipStreamCreateWithFlags(&g_side[k], hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&g_join[k], hipEventDisableTiming));
+    }
+    HIP_OK(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming));
+    g_side_ok = true;
+    return 0;
+}
+
+}  // namespace
+
 int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
 {
+    if (side_streams() != 0) return -1;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, P->dcount, 16);
+    HIP_OK(hipEventRecord(g_fork, stream));
+    const int nside = std::min<int>(kSide, (int)P->launches.size());
+    for (int k = 0; k < nside; ++k) HIP_OK(hipStreamWaitEvent(g_side[k], g_fork, 0));
+    int li = 0;
     for (const Launch &L : P->launches) {
         WaveCfg cfg;
         memset(&cfg, 0, sizeof(cfg));
@@ -701,10 +759,12 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
         cfg.njobs = L.njr;
         cfg.wdims = P->dwd + L.wd0;
         cfg.nwaves = L.nwv;
+        cfg.next_wave = P->dcount + 1 + li;
         cfg.colcode = P->dcc;
         cfg.tab = P->dtab;
-        cfg.tabn = P->tabrows * P->TS;
-        cfg.TS = P->TS;
+        cfg.tabn = P->tabn;
+        cfg.qstride = P->qstride;
+        cfg.ncopy = P->ncopy;
         cfg.VIRT = P->VIRT;
         cfg.PADQ = P->PADQ;
         cfg.PADT = P->PADT;
@@ -717,9 +777,15 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
         cfg.alg = P->alg;
         cfg.sg_flags = P->sg_flags;
         const int wpb = 4;
-        dim3 grid((L.nwv + wpb - 1) / wpb), block(64 * wpb);
-        hipLaunchKernelGGL(kernel_for(L.R, P->sw), grid, block, (size_t)cfg.tabn * 2, stream, cfg);
+        const int blocks = std::min((L.nwv + wpb - 1) / wpb, 1024);
+        hipLaunchKernelGGL(kernel_for(L.R, P->sw), dim3(blocks), dim3(64 * wpb),
+                           (size_t)cfg.tabn * 4, g_side[li % kSide], cfg);
         HIP_OK(hipGetLastError());
+        ++li;
+    }
+    for (int k = 0; k < nside; ++k) {
+        HIP_OK(hipEventRecord(g_join[k], g_side[k]));
+        HIP_OK(hipStreamWaitEvent(stream, g_join[k], 0));
     }
     if (P->nfast) {
         if (P->sw) {
