@@ -68,7 +68,7 @@ def run_resident(h, alg, n, keep=None):
         setattr(br, k, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     if L.parasail_hip_batch_align(h, alg.encode(), 11, 1, ctypes.byref(br)) != 0:
         raise RuntimeError("parasail_hip_batch_align(%s) failed" % alg)
-    return out, P.last_kernel_ms()
+    return out, P.last_kernel_ms(), P.last_main_kernel_ms()
 
 
 def per_call_gcups(reps=1):
@@ -102,7 +102,8 @@ def cpu_baseline(batch, target_s=10.0):
     tests/test_openmp.c:383-441) on a bounded sample of the batch."""
     import oracle as O
     om = O.blosum62()
-    threads = os.cpu_count() or 1
+    # the host's share of cores per GPU is 16, whatever the OS reports
+    threads = max(1, min(os.cpu_count() or 1, 16))
     rng = np.random.default_rng(9)
     idx = rng.choice(len(batch), size=min(len(batch), 200), replace=False)
     algs = {"sw": (O.SW, (0, 0, 0, 0)), "nw": (O.NW, (0, 0, 0, 0)), "sg": (O.SG, (1, 1, 1, 1))}
@@ -128,6 +129,35 @@ def cpu_baseline(batch, target_s=10.0):
                       "OpenMP over pairs, %.1f s" % (n2, secs)}
 
 
+def copy_calibration(mib):
+    """One plain device-to-device copy of a known size: under rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE its counters give the scale from counter
+    KiB to HBM bytes for the kernels of this library."""
+    import torch
+    n = mib << 20
+    src = torch.ones(n // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n}), flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the SW wavefront kernel, as measured by the
+    committed counter profile (two rocprofv3 --pmc passes, FETCH_SIZE and
+    WRITE_SIZE, scaled by a plain copy of known size); None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_wave_sw_config1_hbm_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,7 +169,12 @@ def main():
     ap.add_argument("--uniform", default=None,
                     help="M,N: every pair M x N residues instead of the log-normal mix")
     ap.add_argument("--algs", default="sw,nw,sg")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device once (HBM counter calibration)")
     a = ap.parse_args()
+    if a.copy_calib:
+        copy_calibration(a.copy_calib)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: every configuration of this path fits one MI355X and "
               "no multi-GPU figure is produced (pairs are independent; see the sharding "
@@ -174,11 +209,13 @@ def main():
             run_resident(h, alg, len(batch), bufs[alg])
         log("warmup %d done" % w)
     kms = {alg: [] for alg in algs}
+    mms = {alg: [] for alg in algs}
     t0 = time.perf_counter()
     for s in range(a.steps):
         for alg in algs:
-            _, ms = run_resident(h, alg, len(batch), bufs[alg])
+            _, ms, mm = run_resident(h, alg, len(batch), bufs[alg])
             kms[alg].append(ms)
+            mms[alg].append(mm)
         log("step %d done" % s)
     elapsed = time.perf_counter() - t0
     P.lib().parasail_hip_batch_free(h)
@@ -186,8 +223,10 @@ def main():
     value = len(algs) * cells / (elapsed / a.steps) / 1e9
     per_alg = {alg: round(cells / (np.mean(kms[alg]) / 1e3) / 1e9, 1) for alg in algs}
     a0 = "sw" if "sw" in algs else algs[0]
-    sw_s = np.mean(kms[a0]) / 1e3
-    achieved = cells * OPS[a0] / sw_s
+    # roofline of the dominant kernel: the SW wavefront kernel, one launch
+    # per pass, timed with HIP events on its own stream
+    main_s = np.mean(mms[a0]) / 1e3
+    achieved = cells * OPS[a0] / main_s
     w32 = {alg: int(np.count_nonzero(bufs[alg]["flag"] & P.FLAG_BITS_32)) for alg in algs}
     line = {
         "metric": "GCUPS batched vs per-pair kernel on 1 MI355X, bit-exact; serving p50/p99 ms",
@@ -201,9 +240,11 @@ def main():
         "pairs_int32_tier": w32,
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 2),
                      "peak": round(peak_ops / 1e12, 2), "unit": "TOP/s",
-                     "frac": round(achieved / peak_ops, 4), "traffic": None,
-                     "kernel": "k_wave16 (SW, all length bins)",
-                     "ops_per_cell": OPS["sw"]},
+                     "frac": round(achieved / peak_ops, 4),
+                     "traffic": traffic_from_profile() if a0 == "sw" and not a.uniform else None,
+                     "kernel": "k_wave<%s> (%s wavefront, one launch per pass)"
+                               % ("true" if a0 == "sw" else "false", a0.upper()),
+                     "kernel_ms": round(main_s * 1e3, 3), "ops_per_cell": OPS[a0]},
     }
     if not a.no_percall:
         g, c0, s0 = per_call_gcups()

@@ -399,6 +399,10 @@ void parasail_hip_batch_free(parasail_hip_batch_t *batch);
 
 /* device time (ms, HIP events) of the last call's alignment kernels */
 double parasail_hip_last_kernel_ms(void);
+/* device time (ms, HIP events on its stream) of the last call's dominant
+ * kernel: the wavefront kernel for score-only int16 batches, otherwise the
+ * same as parasail_hip_last_kernel_ms */
+double parasail_hip_last_main_kernel_ms(void);
 const char *parasail_hip_last_error(void);
 /* 1 if a HIP device is usable, 0 otherwise (never falls back to the CPU) */
 int parasail_hip_available(void);

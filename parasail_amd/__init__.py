@@ -119,6 +119,7 @@ def lib():
             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
             ctypes.c_int, ctypes.c_int, MP, ctypes.POINTER(BatchResult)]
         L.parasail_hip_last_kernel_ms.restype = ctypes.c_double
+        L.parasail_hip_last_main_kernel_ms.restype = ctypes.c_double
         L.parasail_hip_last_error.restype = ctypes.c_char_p
         L.parasail_hip_available.restype = ctypes.c_int
         L.parasail_time.restype = ctypes.c_double
@@ -185,6 +186,32 @@ def cigar_tuple(c):
     return out
 
 
+def align_arrays(alg, kind, width, batch, open_, gap, matrix):
+    """Batched alignment of a workloads.PairBatch (concatenated residues with
+    offsets and lengths) through parasail_hip_align_batch; score-only or
+    stats results as numpy arrays."""
+    n = len(batch)
+    res = {k: np.zeros(n, dtype=np.int32) for k in
+           ("score", "end_query", "end_ref", "matches", "similar", "length", "flag")}
+    br = BatchResult()
+    for k in res:
+        setattr(br, k, res[k].ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    I64 = ctypes.POINTER(ctypes.c_int64)
+    I32 = ctypes.POINTER(ctypes.c_int)
+    qoff = np.ascontiguousarray(batch.qoff, dtype=np.int64)
+    toff = np.ascontiguousarray(batch.toff, dtype=np.int64)
+    qlen = np.ascontiguousarray(batch.qlen, dtype=np.int32)
+    tlen = np.ascontiguousarray(batch.tlen, dtype=np.int32)
+    rc = lib().parasail_hip_align_batch(
+        alg.encode(), kind, width, batch.q, qoff.ctypes.data_as(I64), qlen.ctypes.data_as(I32),
+        batch.t, toff.ctypes.data_as(I64), tlen.ctypes.data_as(I32), n, open_, gap, matrix,
+        ctypes.byref(br))
+    if rc != 0:
+        raise RuntimeError("parasail_hip_align_batch failed: %s"
+                           % lib().parasail_hip_last_error().decode(errors="replace"))
+    return res
+
+
 def align_batch(alg, kind, width, queries, targets, open_, gap, matrix, want_cigar=False):
     """Batched alignment through parasail_hip_align_batch.  queries/targets
     are lists of str/bytes (queries may be None for a PSSM).  Returns a dict
@@ -230,4 +257,10 @@ def align_batch(alg, kind, width, queries, targets, open_, gap, matrix, want_cig
 
 
 def last_kernel_ms():
+    """Device time (ms) of the last call's kernels."""
     return lib().parasail_hip_last_kernel_ms()
+
+
+def last_main_kernel_ms():
+    """Device time (ms) of the last call's dominant kernel."""
+    return lib().parasail_hip_last_main_kernel_ms()

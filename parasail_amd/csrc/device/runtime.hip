@@ -26,7 +26,7 @@ static int g_init_rc = -1;
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static hipStream_t g_stream;
 static hipEvent_t g_ev0, g_ev1;
-static double g_last_ms;
+static double g_last_ms, g_last_main_ms;
 
 extern "C" void phip_set_error(const char *fmt, ...)
 {
@@ -38,6 +38,7 @@ extern "C" void phip_set_error(const char *fmt, ...)
 
 extern "C" const char *phip_last_error(void) { return g_err; }
 extern "C" double phip_last_kernel_ms(void) { return g_last_ms; }
+extern "C" double phip_last_main_kernel_ms(void) { return g_last_main_ms; }
 
 static void init_once(void)
 {
@@ -133,6 +134,7 @@ struct phip_resident {
     int8_t *dtr;
     int64_t trtot;
     std::vector<WavePlan *> plans;
+    bool last_wave = false;  // the last pass ran the wavefront tier
 };
 
 typedef struct phip_resident phip_resident_t;
@@ -243,6 +245,7 @@ static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int
 {
     DPParams dp = make_params(r, alg, kind, sg_flags, open, gap, check);
     const bool wave = kind == PHIP_SCORE && !r->mx.is_pssm && !(check && alg != PHIP_SW);
+    r->last_wave = wave;
     if (wave) {
         WavePlan *plan = nullptr;
         for (WavePlan *p : r->plans)
@@ -297,6 +300,7 @@ static int resident_fetch(phip_resident_t *r, phip_out_t *out, int8_t **traces)
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
     g_last_ms = ms;
+    g_last_main_ms = r->last_wave ? wave_last_kernel_ms() : ms;
     if (traces && r->dtr) {
         for (int i = 0; i < r->np; ++i) {
             if (!traces[i]) continue;
@@ -389,7 +393,7 @@ static int banded_locked(const uint8_t *s1, int m, const uint8_t *s2, int n, int
         HIP_OK(hipStreamSynchronize(g_stream));
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
-        g_last_ms = ms;
+        g_last_ms = g_last_main_ms = ms;
         rc = 0;
     } while (0);
     dev_free(base);

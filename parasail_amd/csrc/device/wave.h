@@ -29,3 +29,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
 int wave_plan_run(WavePlan *p, const DPParams &dp, hipStream_t s);
 void wave_plan_free(WavePlan *p);
 bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap);
+// Device time (ms, HIP events on the launch stream) of the wavefront kernel
+// of the last wave_plan_run, once that stream has been synchronised; 0 if
+// the run launched none.
+double wave_last_kernel_ms();

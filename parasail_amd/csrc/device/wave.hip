@@ -796,8 +796,26 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     return P;
 }
 
+namespace {
+hipEvent_t g_kw0, g_kw1;
+bool g_kw_created = false, g_kw_recorded = false;
+}  // namespace
+
+double wave_last_kernel_ms()
+{
+    float ms = 0.f;
+    if (!g_kw_recorded || hipEventElapsedTime(&ms, g_kw0, g_kw1) != hipSuccess) return 0.0;
+    return ms;
+}
+
 int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
 {
+    if (!g_kw_created) {
+        HIP_OK(hipEventCreate(&g_kw0));
+        HIP_OK(hipEventCreate(&g_kw1));
+        g_kw_created = true;
+    }
+    g_kw_recorded = false;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, P->dcount, 16);
     if (P->nwaves) {
         WaveCfg cfg;
@@ -823,8 +841,11 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
         cfg.alg = P->alg;
         cfg.sg_flags = P->sg_flags;
         KFn fn = P->sw ? k_wave<true> : k_wave<false>;
+        HIP_OK(hipEventRecord(g_kw0, stream));
         hipLaunchKernelGGL(fn, dim3(P->nblocks), dim3(256), (size_t)cfg.tabn * 4, stream, cfg);
         HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(g_kw1, stream));
+        g_kw_recorded = true;
         if (P->sw) {
             hipLaunchKernelGGL(k_sw_reduce, dim3((P->nfast + 255) / 256), dim3(256), 0, stream,
                                P->dcd, P->drefs, P->nfast, P->ovf_limit, dp.out, P->dredo,

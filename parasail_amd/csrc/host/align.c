@@ -404,6 +404,7 @@ PHOST_EXPORT parasail_result_t *parasail_nw_banded_hip(const char *const s1, con
 /* ---- misc ---- */
 
 PHOST_EXPORT double parasail_hip_last_kernel_ms(void) { return phip_last_kernel_ms(); }
+PHOST_EXPORT double parasail_hip_last_main_kernel_ms(void) { return phip_last_main_kernel_ms(); }
 PHOST_EXPORT const char *parasail_hip_last_error(void) { return phip_last_error(); }
 PHOST_EXPORT int parasail_hip_available(void) { return phip_init() == 0; }
 PHOST_EXPORT int parasail_hip_device_info(int *compute_units, int *clock_khz)

@@ -90,6 +90,9 @@ int phip_device_info(int *cus, int *clock_khz);
 
 /* timing of the last phip_align call's dominant kernel (ms, HIP events) */
 double phip_last_kernel_ms(void);
+/* device time of the last call's dominant kernel (the wavefront kernel when
+ * it ran, else the same as phip_last_kernel_ms) */
+double phip_last_main_kernel_ms(void);
 
 #ifdef __cplusplus
 }

@@ -302,3 +302,61 @@ def test_banded(mats):
             assert (rc.score, rc.end_query, rc.end_ref) == exp[:3]
             assert rc.flag & P.FLAG_BANDED and rc.flag & P.FLAG_NW
             L.parasail_result_free(r)
+
+
+def _oracle_pairs(batch, idx, name, om):
+    alg, flags = oracle_alg(name)
+    qs = [om.map(batch.query(int(k))) for k in idx]
+    ts = [om.map(batch.target(int(k))) for k in idx]
+    exp, _ = O.batch(alg, O.SCORE, qs, ts, 11, 1, om, flags)
+    return exp
+
+
+def test_batch_composition_independent(mats):
+    """A pair's result does not depend on the rest of its batch: the planner
+    picks rows per lane, chain width and queue order from the whole batch."""
+    from parasail_amd import workloads as WL
+    pm, om = mats["blosum62"]
+    b = WL.config1(seed=11, npairs=3000)
+    sub = np.sort(np.random.default_rng(12).choice(len(b), 300, replace=False))
+    bs = b.subset(sub)
+    for name in ("sw", "nw", "sg"):
+        full = P.align_arrays(name, 0, 0, b, 11, 1, pm)
+        part = P.align_arrays(name, 0, 0, bs, 11, 1, pm)
+        for key in ("score", "end_query", "end_ref"):
+            assert (full[key][sub] == part[key]).all(), (name, key)
+        exp = _oracle_pairs(bs, range(len(bs)), name, om)
+        got = np.stack([part["score"], part["end_query"], part["end_ref"]], axis=1)
+        bad = np.nonzero((got != exp[:, :3]).any(axis=1))[0]
+        assert bad.size == 0, (name, [(int(sub[k]), got[k].tolist(), exp[k, :3].tolist())
+                                      for k in bad[:5]])
+
+
+def test_config1_full_size_properties(mats):
+    """The full 200,000-pair protein batch (log-normal lengths around 300,
+    BLOSUM62 11/1): a random 2,000-pair sample equals the oracle bit for bit;
+    on every pair SW >= SG (all ends free) >= NW, SW >= 0, the end positions
+    lie inside the sequences, and a second pass reproduces the first."""
+    from parasail_amd import workloads as WL
+    pm, om = mats["blosum62"]
+    b = WL.config1()
+    n = len(b)
+    res = {name: P.align_arrays(name, 0, 0, b, 11, 1, pm) for name in ("sw", "nw", "sg")}
+    sw, nw, sg = res["sw"]["score"], res["nw"]["score"], res["sg"]["score"]
+    assert (sw >= 0).all() and (sw >= sg).all() and (sg >= nw).all()
+    for name in ("sw", "nw", "sg"):
+        r = res[name]
+        assert ((r["end_query"] >= 0) & (r["end_query"] < b.qlen)).all(), name
+        assert ((r["end_ref"] >= 0) & (r["end_ref"] < b.tlen)).all(), name
+        assert not (r["flag"] & P.FLAG_SATURATED).any(), name
+    again = P.align_arrays("sw", 0, 0, b, 11, 1, pm)
+    for key in ("score", "end_query", "end_ref"):
+        assert (again[key] == res["sw"][key]).all()
+    idx = np.sort(np.random.default_rng(13).choice(n, 2000, replace=False))
+    for name in ("sw", "nw", "sg"):
+        exp = _oracle_pairs(b, idx, name, om)
+        r = res[name]
+        got = np.stack([r["score"][idx], r["end_query"][idx], r["end_ref"][idx]], axis=1)
+        bad = np.nonzero((got != exp[:, :3]).any(axis=1))[0]
+        assert bad.size == 0, (name, [(int(idx[k]), got[k].tolist(), exp[k, :3].tolist())
+                                      for k in bad[:5]])
