@@ -174,9 +174,12 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         if (b == 0 && t == 0) prevRH = pack16(-open, -open);  // H[0][0] = 0
         int botH = PK_NEG, botF = PK_NEG;
         int best = (int)0xFFFFFFFF, bcol = 0;
-        int snap[SW ? R : 1];
+        // SW: the lane's rows at the column of its running maximum; NW/SG:
+        // the lane's rows at each half's last column
+        int snap[R];
 #pragma unroll
-        for (int i = 0; i < (SW ? R : 1); ++i) snap[i] = 0;
+        for (int i = 0; i < R; ++i) snap[i] = 0;
+        const int lastA = J.n[0] - 1, lastB = J.n[1] - 1;
 
         // column codes: ccl[k] is the code pair of column k - t; the stream
         // holds T leading and 8 trailing pad codes around the wave's width
@@ -234,22 +237,16 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                         const int d = nbst ^ best;
                         const int mask = ((d & 0xffff) ? 0xffff : 0) | ((d >> 16) ? (int)0xffff0000 : 0);
 #pragma unroll
-                        for (int i = 0; i < (SW ? R : 1); ++i) snap[i] = (HL[i] & mask) | (snap[i] & ~mask);
+                        for (int i = 0; i < R; ++i) snap[i] = (HL[i] & mask) | (snap[i] & ~mask);
                         bcol = (pack16(j, j) & mask) | (bcol & ~mask);
                         best = nbst;
                     }
                 } else {
-                    // the last column: largest value over rows 0..m-2, first row
+                    const bool ha = j == lastA, hb = j == lastB;
+                    if (ha || hb) {
+                        const int mask = (ha ? 0xffff : 0) | (hb ? (int)0xffff0000 : 0);
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        if (j == J.n[h] - 1 && band_live) {
-#pragma unroll
-                            for (int i = 0; i < R; ++i) {
-                                const int g = first_row[h] + i;
-                                const int v = (h ? hi16(HL[i]) : lo16(HL[i])) + open;
-                                if (g >= 0 && g < J.m[h] - 1 && v > cmx[h]) { cmx[h] = v; crow[h] = g; }
-                            }
-                        }
+                        for (int i = 0; i < R; ++i) snap[i] = (HL[i] & mask) | (snap[i] & ~mask);
                     }
                 }
             }
@@ -270,13 +267,25 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 const int col = h ? hi16(bcol) : lo16(bcol);
                 int row = -1;
 #pragma unroll
-                for (int i = (SW ? R : 1) - 1; i >= 0; --i) {
+                for (int i = R - 1; i >= 0; --i) {
                     const int g = first_row[h] + i;
                     const int v = (h ? hi16(snap[i]) : lo16(snap[i])) + open;
                     if (g >= 0 && g < J.m[h] && v == S) row = g;
                 }
                 if (row >= 0 && (S > cS[h] || (S == cS[h] && col < cC[h]))) {
                     cS[h] = S; cC[h] = col; cR[h] = row;
+                }
+            }
+        }
+        if (!SW && band_live) {
+            // the last column: largest value over rows 0..m-2, first row
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const int g = first_row[h] + i;
+                    const int v = (h ? hi16(snap[i]) : lo16(snap[i])) + open;
+                    if (g >= 0 && g < J.m[h] - 1 && v > cmx[h]) { cmx[h] = v; crow[h] = g; }
                 }
             }
         }
@@ -320,9 +329,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         if (wd.prio) __builtin_amdgcn_s_setprio(2);
         switch (wd.R) {
         case 4: wave_run<4, SW>(c, wd, tabb, lane, dump); break;
+        case 6: wave_run<6, SW>(c, wd, tabb, lane, dump); break;
         case 8: wave_run<8, SW>(c, wd, tabb, lane, dump); break;
+        case 10: wave_run<10, SW>(c, wd, tabb, lane, dump); break;
         case 12: wave_run<12, SW>(c, wd, tabb, lane, dump); break;
+        case 14: wave_run<14, SW>(c, wd, tabb, lane, dump); break;
         case 16: wave_run<16, SW>(c, wd, tabb, lane, dump); break;
+        case 18: wave_run<18, SW>(c, wd, tabb, lane, dump); break;
         default: wave_run<20, SW>(c, wd, tabb, lane, dump); break;
         }
         if (wd.prio) __builtin_amdgcn_s_setprio(0);
@@ -445,7 +458,7 @@ __global__ void k_colcode(const WaveJob *jobs, const int *clen, const int *cT, i
 
 namespace {
 
-const int kRs[] = {4, 8, 12, 16, 20};
+const int kRs[] = {4, 6, 8, 10, 12, 14, 16, 18, 20};
 constexpr int kRmax = 20;
 
 struct Shape {
