@@ -69,6 +69,8 @@ __device__ __forceinline__ int shr1(int v)
 }  // namespace
 
 #define PK_NEG ((int)0x80008000)
+// largest rows per lane (the SW snapshot area in LDS is sized for it)
+constexpr int kSnapRows = 20;
 #define W_NEG (-12000)
 
 // One job (two pairs).  Host-built, device-read.
@@ -119,7 +121,7 @@ namespace {
 // All bands of one queue entry, R rows per lane.
 template <int R, bool SW>
 __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, const char *tabb,
-                                         int lane, int2 *dump)
+                                         int lane, int2 *dump, short *snp)
 {
     const int T = wd.T;
     const int t = lane & (T - 1);
@@ -173,12 +175,13 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         int prevRH = pack16(left(first_row[0] - 1), left(first_row[1] - 1));
         if (b == 0 && t == 0) prevRH = pack16(-open, -open);  // H[0][0] = 0
         int botH = PK_NEG, botF = PK_NEG;
-        int best = (int)0xFFFFFFFF, bcol = 0;
-        // SW: the lane's rows at the column of its running maximum; NW/SG:
-        // the lane's rows at each half's last column
-        int snap[R];
+        int best = (int)0xFFFFFFFF;
+        // NW/SG: the lane's rows at each half's last column (SW keeps its
+        // snapshot, the rows at the column of the running maximum, in LDS)
+        int snap[SW ? 1 : R];
 #pragma unroll
-        for (int i = 0; i < R; ++i) snap[i] = 0;
+        for (int i = 0; i < (SW ? 1 : R); ++i) snap[i] = 0;
+        int bcolA = 0, bcolB = 0;
         const int lastA = J.n[0] - 1, lastB = J.n[1] - 1;
 
         // column codes: ccl[k] is the code pair of column k - t; the stream
@@ -234,11 +237,19 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 if (SW) {
                     const int nbst = pk_max(best, cm);
                     if (nbst != best) {
+                        // the half whose maximum rose stores its rows in LDS
+                        // (per-thread slots, rows 256 slots apart)
                         const int d = nbst ^ best;
-                        const int mask = ((d & 0xffff) ? 0xffff : 0) | ((d >> 16) ? (int)0xffff0000 : 0);
+                        if (d & 0xffff) {
 #pragma unroll
-                        for (int i = 0; i < R; ++i) snap[i] = (HL[i] & mask) | (snap[i] & ~mask);
-                        bcol = (pack16(j, j) & mask) | (bcol & ~mask);
+                            for (int i = 0; i < R; ++i) snp[i * 256] = (short)HL[i];
+                            bcolA = j;
+                        }
+                        if (d >> 16) {
+#pragma unroll
+                            for (int i = 0; i < R; ++i) snp[(kSnapRows + i) * 256] = (short)(HL[i] >> 16);
+                            bcolB = j;
+                        }
                         best = nbst;
                     }
                 } else {
@@ -264,12 +275,12 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
             for (int h = 0; h < 2; ++h) {
                 if (J.pair[h] < 0) continue;
                 const int S = h ? hi16(best) : lo16(best);
-                const int col = h ? hi16(bcol) : lo16(bcol);
+                const int col = h ? bcolB : bcolA;
                 int row = -1;
 #pragma unroll
                 for (int i = R - 1; i >= 0; --i) {
                     const int g = first_row[h] + i;
-                    const int v = (h ? hi16(snap[i]) : lo16(snap[i])) + open;
+                    const int v = (int)snp[(h * kSnapRows + i) * 256] + open;
                     if (g >= 0 && g < J.m[h] && v == S) row = g;
                 }
                 if (row >= 0 && (S > cS[h] || (S == cS[h] && col < cC[h]))) {
@@ -309,6 +320,9 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
 // W(q,t)+open sits at byte  q*qstride + (t>>1)*128*(ncopy/32) + (t&1)*2 + c*4
 // with c = lane % ncopy: with 32 copies every lane of a half-wave reads its
 // own bank, so the random (q, t) lookups of a wave never conflict.
+//
+// SW end snapshots follow the table in LDS: 2 x kSnapRows rows of 256
+// int16 slots, one per thread (half A rows, then half B rows).
 template <bool SW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_wave(WaveCfg c)
 {
@@ -317,6 +331,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const char *tabb = (const char *)tab32 + (lane % c.ncopy) * 4;
+    short *const snp = (short *)(tab32 + ((c.tabn + 3) & ~3)) + threadIdx.x;
     int2 *const dump = c.dump + (threadIdx.x + blockIdx.x * blockDim.x);
     // persistent: each wave takes the next entry off the queue (entries are
     // ordered by decreasing latency, so the tail is short)
@@ -328,15 +343,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         const WaveDesc wd = c.waves[wv];
         if (wd.prio) __builtin_amdgcn_s_setprio(2);
         switch (wd.R) {
-        case 4: wave_run<4, SW>(c, wd, tabb, lane, dump); break;
-        case 6: wave_run<6, SW>(c, wd, tabb, lane, dump); break;
-        case 8: wave_run<8, SW>(c, wd, tabb, lane, dump); break;
-        case 10: wave_run<10, SW>(c, wd, tabb, lane, dump); break;
-        case 12: wave_run<12, SW>(c, wd, tabb, lane, dump); break;
-        case 14: wave_run<14, SW>(c, wd, tabb, lane, dump); break;
-        case 16: wave_run<16, SW>(c, wd, tabb, lane, dump); break;
-        case 18: wave_run<18, SW>(c, wd, tabb, lane, dump); break;
-        default: wave_run<20, SW>(c, wd, tabb, lane, dump); break;
+        case 4: wave_run<4, SW>(c, wd, tabb, lane, dump, snp); break;
+        case 6: wave_run<6, SW>(c, wd, tabb, lane, dump, snp); break;
+        case 8: wave_run<8, SW>(c, wd, tabb, lane, dump, snp); break;
+        case 10: wave_run<10, SW>(c, wd, tabb, lane, dump, snp); break;
+        case 12: wave_run<12, SW>(c, wd, tabb, lane, dump, snp); break;
+        case 14: wave_run<14, SW>(c, wd, tabb, lane, dump, snp); break;
+        case 16: wave_run<16, SW>(c, wd, tabb, lane, dump, snp); break;
+        case 18: wave_run<18, SW>(c, wd, tabb, lane, dump, snp); break;
+        default: wave_run<20, SW>(c, wd, tabb, lane, dump, snp); break;
         }
         if (wd.prio) __builtin_amdgcn_s_setprio(0);
     }
@@ -458,7 +473,7 @@ __global__ void k_colcode(const WaveJob *jobs, const int *clen, const int *cT, i
 
 namespace {
 
-const int kRs[] = {4, 6, 8, 10, 12, 14, 16, 18, 20};
+const int kRs[] = {4, 6, 8, 10, 12, 14, 16, 18, kSnapRows};
 constexpr int kRmax = 20;
 
 struct Shape {
@@ -855,7 +870,8 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
         cfg.sg_flags = P->sg_flags;
         KFn fn = P->sw ? k_wave<true> : k_wave<false>;
         HIP_OK(hipEventRecord(g_kw0, stream));
-        hipLaunchKernelGGL(fn, dim3(P->nblocks), dim3(256), (size_t)cfg.tabn * 4, stream, cfg);
+        const size_t lds = (size_t)((cfg.tabn + 3) & ~3) * 4 + (P->sw ? 2 * kSnapRows * 256 * 2 : 0);
+        hipLaunchKernelGGL(fn, dim3(P->nblocks), dim3(256), lds, stream, cfg);
         HIP_OK(hipGetLastError());
         HIP_OK(hipEventRecord(g_kw1, stream));
         g_kw_recorded = true;
