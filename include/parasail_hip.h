@@ -397,6 +397,51 @@ int parasail_hip_batch_align(parasail_hip_batch_t *batch, const char *alg_name, 
                              int gap, parasail_hip_batch_result_t *out);
 void parasail_hip_batch_free(parasail_hip_batch_t *batch);
 
+/* ---- database search and serving (no reference counterpart; the
+ * reference's callers run parasail_sw_striped_profile_* over every target
+ * of a database in an OpenMP loop, tests/test_openmp.c:353-369) ----
+ *
+ * A target database kept resident in device memory (HBM): residues are
+ * mapped with the matrix's mapper and uploaded once.  A search is SW of one
+ * query against every target (int16, with an exact int32 re-run of any
+ * target whose score leaves the int16 window); it returns the k best hits,
+ * score descending then target index ascending, with the end positions
+ * parasail_sw would report.  The query is s1 with a square matrix, or a
+ * PSSM (parasail_matrix_pssm_create; s1 may be NULL) whose alphabet is the
+ * database's. */
+typedef struct parasail_hip_db parasail_hip_db_t;
+typedef struct parasail_hip_hit {
+    int score, end_query, end_ref, target;
+} parasail_hip_hit_t;
+
+parasail_hip_db_t *parasail_hip_db_create(const char *seqs, const int64_t *off, const int *len,
+                                          int n, const parasail_matrix_t *matrix);
+void parasail_hip_db_free(parasail_hip_db_t *db);
+/* number of hits written (min(k, targets)), or -1 */
+int parasail_hip_db_search(parasail_hip_db_t *db, const char *s1, int s1Len,
+                           const parasail_matrix_t *matrix, int open, int gap, int k,
+                           parasail_hip_hit_t *hits);
+
+/* Serving loop over a database: requests are queued by submit and run by a
+ * background thread with up to four in flight (one HIP stream each), each
+ * admitted when a slot and mem_budget bytes of device memory (0: 32 GiB)
+ * allow; in-flight requests share the GPU chunk by chunk.  arrival is the
+ * request's arrival time on parasail_hip_now()'s clock (0: now); its latency
+ * is measured from there to the moment its hits are back in host memory.
+ * poll returns 1 with the next finished request (hits[] needs room for k),
+ * 0 if none is finished (wait = 0), -1 if that request failed. */
+typedef struct parasail_hip_server parasail_hip_server_t;
+
+parasail_hip_server_t *parasail_hip_server_create(parasail_hip_db_t *db, size_t mem_budget);
+int parasail_hip_server_submit(parasail_hip_server_t *srv, int64_t id, const char *s1, int s1Len,
+                               const parasail_matrix_t *matrix, int open, int gap, int k,
+                               double arrival);
+int parasail_hip_server_poll(parasail_hip_server_t *srv, int64_t *id, parasail_hip_hit_t *hits,
+                             int *nhits, double *latency_ms, int wait);
+void parasail_hip_server_free(parasail_hip_server_t *srv);
+/* monotonic clock (seconds) of the serving loop */
+double parasail_hip_now(void);
+
 /* device time (ms, HIP events) of the last call's alignment kernels */
 double parasail_hip_last_kernel_ms(void);
 /* device time (ms, HIP events on its stream) of the last call's dominant

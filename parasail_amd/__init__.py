@@ -62,6 +62,14 @@ class Cigar(ctypes.Structure):
                 ("beg_query", ctypes.c_int), ("beg_ref", ctypes.c_int)]
 
 
+class Hit(ctypes.Structure):
+    _fields_ = [("score", ctypes.c_int), ("end_query", ctypes.c_int), ("end_ref", ctypes.c_int),
+                ("target", ctypes.c_int)]
+
+    def as_tuple(self):
+        return (self.score, self.end_query, self.end_ref, self.target)
+
+
 class BatchResult(ctypes.Structure):
     _fields_ = [("score", ctypes.POINTER(ctypes.c_int)), ("end_query", ctypes.POINTER(ctypes.c_int)),
                 ("end_ref", ctypes.POINTER(ctypes.c_int)), ("matches", ctypes.POINTER(ctypes.c_int)),
@@ -119,6 +127,26 @@ def lib():
             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
             ctypes.c_int, ctypes.c_int, MP, ctypes.POINTER(BatchResult)]
         L.parasail_hip_last_kernel_ms.restype = ctypes.c_double
+        I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
+        L.parasail_hip_db_create.restype = ctypes.c_void_p
+        L.parasail_hip_db_create.argtypes = [ctypes.c_char_p, I64, I32, ctypes.c_int, MP]
+        L.parasail_hip_db_free.argtypes = [ctypes.c_void_p]
+        L.parasail_hip_db_search.restype = ctypes.c_int
+        L.parasail_hip_db_search.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, MP,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(Hit)]
+        L.parasail_hip_server_create.restype = ctypes.c_void_p
+        L.parasail_hip_server_create.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        L.parasail_hip_server_submit.restype = ctypes.c_int
+        L.parasail_hip_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_char_p,
+                                                 ctypes.c_int, MP, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_double]
+        L.parasail_hip_server_poll.restype = ctypes.c_int
+        L.parasail_hip_server_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(Hit), ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.parasail_hip_server_free.argtypes = [ctypes.c_void_p]
+        L.parasail_hip_now.restype = ctypes.c_double
         L.parasail_hip_last_main_kernel_ms.restype = ctypes.c_double
         L.parasail_hip_last_error.restype = ctypes.c_char_p
         L.parasail_hip_available.restype = ctypes.c_int
@@ -264,3 +292,78 @@ def last_kernel_ms():
 def last_main_kernel_ms():
     """Device time (ms) of the last call's dominant kernel."""
     return lib().parasail_hip_last_main_kernel_ms()
+
+
+class Database:
+    """An HBM-resident target database (parasail_hip_db_*): a
+    workloads.PairBatch-like object's targets, or concatenated residues with
+    offsets and lengths."""
+
+    def __init__(self, residues, offsets, lengths, matrix):
+        self._off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._len = np.ascontiguousarray(lengths, dtype=np.int32)
+        self.n = len(self._len)
+        self.cells_per_row = int(self._len.astype(np.int64).sum())
+        self.h = lib().parasail_hip_db_create(
+            residues, self._off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            self._len.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), self.n, matrix)
+        if not self.h:
+            raise RuntimeError("parasail_hip_db_create failed: %s"
+                               % lib().parasail_hip_last_error().decode(errors="replace"))
+
+    def search(self, query, matrix, open_, gap, k):
+        """SW of query (str/bytes, or None with a PSSM) against every target:
+        list of (score, end_query, end_ref, target), best first."""
+        hits = (Hit * k)()
+        q = _b(query) if query is not None else None
+        m = len(q) if q is not None else 0
+        n = lib().parasail_hip_db_search(self.h, q, m, matrix, open_, gap, k, hits)
+        if n < 0:
+            raise RuntimeError("parasail_hip_db_search failed: %s"
+                               % lib().parasail_hip_last_error().decode(errors="replace"))
+        return [hits[i].as_tuple() for i in range(n)]
+
+    def close(self):
+        if self.h:
+            lib().parasail_hip_db_free(self.h)
+            self.h = None
+
+
+class Server:
+    """The serving loop over a Database (parasail_hip_server_*)."""
+
+    def __init__(self, db, mem_budget=0):
+        self.db = db
+        self.h = lib().parasail_hip_server_create(db.h, mem_budget)
+        if not self.h:
+            raise RuntimeError("parasail_hip_server_create failed")
+
+    def submit(self, rid, query, matrix, open_, gap, k, arrival=0.0):
+        q = _b(query) if query is not None else None
+        m = len(q) if q is not None else 0
+        if lib().parasail_hip_server_submit(self.h, rid, q, m, matrix, open_, gap, k, arrival) != 0:
+            raise RuntimeError("parasail_hip_server_submit failed")
+
+    def poll(self, k, wait=True):
+        """(id, hits, latency_ms) of the next finished request, or None."""
+        rid = ctypes.c_int64()
+        hits = (Hit * k)()
+        nh = ctypes.c_int()
+        lat = ctypes.c_double()
+        rc = lib().parasail_hip_server_poll(self.h, ctypes.byref(rid), hits, ctypes.byref(nh),
+                                            ctypes.byref(lat), 1 if wait else 0)
+        if rc == 0:
+            return None
+        if rc < 0:
+            raise RuntimeError("request %d failed" % rid.value)
+        return rid.value, [hits[i].as_tuple() for i in range(nh.value)], lat.value
+
+    def close(self):
+        if self.h:
+            lib().parasail_hip_server_free(self.h)
+            self.h = None
+
+
+def now():
+    """The serving loop's clock (seconds)."""
+    return lib().parasail_hip_now()

@@ -16,8 +16,9 @@ struct DPParams {
     const int *matrix;     // device, row-major length*size
     int msize, is_pssm;
     const uint8_t *q;      // device residue codes
-    const int64_t *qoff;
-    const int32_t *qlen;
+    const int64_t *qoff;   // NULL: one query (offset 0) for every pair
+    const int32_t *qlen;   // NULL: every pair's query has qlen1 rows
+    int qlen1;
     const uint8_t *t;
     const int64_t *toff;
     const int32_t *tlen;

@@ -50,8 +50,8 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
     const int s2_beg = (alg == PHIP_SG) && (p.sg_flags & PHIP_S2_BEG);
     const int s2_end = (alg == PHIP_SG) && (p.sg_flags & PHIP_S2_END);
     const int open = p.open, gap = p.gap;
-    const int m = p.qlen[pr], n = p.tlen[pr];
-    const uint8_t *s1 = p.q ? p.q + p.qoff[pr] : nullptr;
+    const int m = p.qlen ? p.qlen[pr] : p.qlen1, n = p.tlen[pr];
+    const uint8_t *s1 = p.q ? p.q + (p.qoff ? p.qoff[pr] : 0) : nullptr;
     const uint8_t *s2 = p.t + p.toff[pr];
     const int *mat = p.matrix;
     const int msize = p.msize;

@@ -37,6 +37,10 @@ extern "C" void phip_set_error(const char *fmt, ...)
 }
 
 extern "C" const char *phip_last_error(void) { return g_err; }
+
+void phip_lock() { pthread_mutex_lock(&g_lock); }
+void phip_unlock() { pthread_mutex_unlock(&g_lock); }
+hipStream_t phip_stream() { return g_stream; }
 extern "C" double phip_last_kernel_ms(void) { return g_last_ms; }
 extern "C" double phip_last_main_kernel_ms(void) { return g_last_main_ms; }
 

@@ -88,6 +88,46 @@ int phip_nw_banded(const uint8_t *s1, int m, const uint8_t *s2, int n,
 
 int phip_device_info(int *cus, int *clock_khz);
 
+/* ---- database search and serving (search.hip) ---- */
+
+typedef struct phip_hit {
+    int score, end_query, end_ref, target;
+} phip_hit_t;
+
+/* one query: residue codes (NULL for a PSSM) and its scoring */
+typedef struct phip_query {
+    const uint8_t *codes;
+    int m;
+    phip_matrix_t mx;   /* square (codes index rows) or PSSM (m rows) */
+    int open, gap;
+} phip_query_t;
+
+typedef struct phip_db phip_db_t;
+typedef struct phip_server phip_server_t;
+
+/* targets: mapped codes with per-target offsets/lengths; size = alphabet
+ * columns of the matrices that will search it (< 256) */
+phip_db_t *phip_db_create(const uint8_t *codes, const int64_t *off, const int32_t *len, int n,
+                          int size);
+void phip_db_free(phip_db_t *db);
+int64_t phip_db_residues(const phip_db_t *db);
+/* SW of the query against every target; the k best (score desc, target
+ * index asc) into hits; *nhits = min(k, targets).  0 on success. */
+int phip_db_search(phip_db_t *db, const phip_query_t *q, int k, phip_hit_t *hits, int *nhits);
+
+/* serving loop: up to 4 requests in flight (one stream each), admitted as
+ * slots and device memory (mem_budget bytes, 0 = 32 GiB) free up */
+phip_server_t *phip_server_create(phip_db_t *db, size_t mem_budget);
+/* queues a request (inputs are copied); arrival = phip_now() time of its
+ * arrival (0 = now), from which its latency is measured */
+int phip_server_submit(phip_server_t *s, int64_t id, const phip_query_t *q, int k, double arrival);
+/* next finished request: 1 with its hits and latency, 0 if none (wait = 0),
+ * -1 if it failed */
+int phip_server_poll(phip_server_t *s, int64_t *id, phip_hit_t *hits, int *nhits,
+                     double *latency_ms, int wait);
+void phip_server_free(phip_server_t *s);
+double phip_now(void);
+
 /* timing of the last phip_align call's dominant kernel (ms, HIP events) */
 double phip_last_kernel_ms(void);
 /* device time of the last call's dominant kernel (the wavefront kernel when
