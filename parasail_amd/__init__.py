@@ -367,3 +367,28 @@ class Server:
 def now():
     """The serving loop's clock (seconds)."""
     return lib().parasail_hip_now()
+
+
+PSSM_ALPHABET = b"ARNDCQEGHILKMFPSTWYVBZX"
+
+
+def blosum62_pssm(query, noise=None):
+    """A PSSM (parasail_matrix_pssm_create) whose row i is the BLOSUM62 row
+    of query[i], plus noise[i] (ints, len(query) x 23) when given, over the
+    alphabet ARNDCQEGHILKMFPSTWYVBZX (the BLOSUM62 mapper).  Returns the
+    matrix and the values array, which must outlive nothing (the matrix
+    copies them) but is returned for inspection."""
+    b = blosum62().contents
+    q = _b(query)
+    idx = np.array([b.mapper[c] for c in PSSM_ALPHABET])
+    tab = np.ctypeslib.as_array(b.matrix, shape=(b.size * b.size,)).reshape(b.size, b.size)
+    rows = np.array([b.mapper[c] for c in q])
+    vals = tab[rows][:, idx].astype(np.int32)
+    if noise is not None:
+        vals = vals + np.asarray(noise, dtype=np.int32)
+    v = np.ascontiguousarray(vals.reshape(-1))
+    m = lib().parasail_matrix_pssm_create(PSSM_ALPHABET,
+                                          v.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(q))
+    if not m:
+        raise RuntimeError("parasail_matrix_pssm_create failed")
+    return m, v

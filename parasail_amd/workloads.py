@@ -116,3 +116,42 @@ def config2(seed=2, npairs=100000):
         reads.append(r)
         rl.append(len(r))
     return PairBatch(b"".join(reads), rl, win.tobytes(), np.full(npairs, 400))
+
+
+def protein_bytes_fast(rng, total):
+    """protein_bytes for database-sized draws: residues from a 65,536-entry
+    table holding each amino acid in proportion to its background frequency
+    (frequencies rounded to 1/65,536)."""
+    p = RR_PER_1000 / RR_PER_1000.sum()
+    counts = np.floor(p * 65536).astype(np.int64)
+    counts[np.argmax(counts)] += 65536 - counts.sum()
+    table = np.repeat(_AA_BYTES, counts)
+    return table[rng.integers(0, 65536, size=int(total), dtype=np.uint16)].tobytes()
+
+
+def config4_db(seed=4, ntargets=1000000):
+    """The serving database: proteins with log-normal lengths (median 350,
+    sigma 0.7, clipped to [30, 35000])."""
+    rng = np.random.default_rng(seed)
+    tl = lognormal_lengths(rng, ntargets, 350, 0.7, 30, 35000)
+    return protein_bytes_fast(rng, int(tl.sum())), concat_offsets(tl), tl.astype(np.int32)
+
+
+def config4_queries(seed=5, nreq=64, qps=10.0, pssm_frac=0.1):
+    """Serving requests: (arrival offset s, query bytes, pssm values or None)
+    with Poisson arrivals at qps, log-normal lengths (median 300, sigma 0.6,
+    clipped to [20, 5000]); a pssm_frac share carries a PSSM of per-position
+    BLOSUM62 rows with uniform integer noise in [-2, 2], given as the int
+    values of the 23-letter alphabet ARNDCQEGHILKMFPSTWYVBZX per position."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / qps, size=nreq)
+    arr = np.cumsum(gaps) - gaps[0]
+    ql = lognormal_lengths(rng, nreq, 300, 0.6, 20, 5000)
+    out = []
+    for k in range(nreq):
+        q = protein_bytes(rng, int(ql[k]))
+        pssm = None
+        if rng.random() < pssm_frac:
+            pssm = rng.integers(-2, 3, size=(int(ql[k]), 23)).astype(np.int32)  # noise
+        out.append((float(arr[k]), q, pssm))
+    return out
