@@ -383,6 +383,15 @@ int parasail_hip_align_batch(const char *alg_name, int kind, int width, const ch
                              int gap, const parasail_matrix_t *matrix,
                              parasail_hip_batch_result_t *out);
 
+/* Banded global alignment (parasail_nw_banded semantics, band k) of npairs
+ * pairs in one device pass; score and end positions into out (the other
+ * arrays of out are zeroed).  0, or -1 with a message. */
+int parasail_hip_nw_banded_batch(const char *s1, const int64_t *s1_off, const int *s1_len,
+                                 const char *s2, const int64_t *s2_off, const int *s2_len,
+                                 int npairs, int open, int gap, int k,
+                                 const parasail_matrix_t *matrix,
+                                 parasail_hip_batch_result_t *out);
+
 /* A batch kept resident in device memory (HBM): the residues are mapped and
  * uploaded once, and every parasail_hip_batch_align call over it launches
  * kernels only.  Score-only (kind 0) results, as parasail_hip_align_batch

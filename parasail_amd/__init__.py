@@ -392,3 +392,35 @@ def blosum62_pssm(query, noise=None):
     if not m:
         raise RuntimeError("parasail_matrix_pssm_create failed")
     return m, v
+
+
+def nw_banded_batch(queries, targets, open_, gap, k, matrix):
+    """parasail_hip_nw_banded_batch over lists of str/bytes: score,
+    end_query, end_ref and flag arrays."""
+    n = len(queries)
+    qb = [_b(q) for q in queries]
+    tb = [_b(t) for t in targets]
+    ql = np.array([len(x) for x in qb], dtype=np.int32)
+    tl = np.array([len(x) for x in tb], dtype=np.int32)
+    qo = np.zeros(n, dtype=np.int64)
+    to = np.zeros(n, dtype=np.int64)
+    if n:
+        qo[1:] = np.cumsum(ql, dtype=np.int64)[:-1]
+        to[1:] = np.cumsum(tl, dtype=np.int64)[:-1]
+    res = {k_: np.zeros(n, dtype=np.int32) for k_ in ("score", "end_query", "end_ref", "flag")}
+    br = BatchResult()
+    for k_, v in res.items():
+        setattr(br, k_, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
+    L = lib()
+    L.parasail_hip_nw_banded_batch.restype = ctypes.c_int
+    L.parasail_hip_nw_banded_batch.argtypes = [ctypes.c_char_p, I64, I32, ctypes.c_char_p, I64, I32,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, MP, ctypes.POINTER(BatchResult)]
+    rc = L.parasail_hip_nw_banded_batch(b"".join(qb), qo.ctypes.data_as(I64), ql.ctypes.data_as(I32),
+                                        b"".join(tb), to.ctypes.data_as(I64), tl.ctypes.data_as(I32),
+                                        n, open_, gap, k, matrix, ctypes.byref(br))
+    if rc != 0:
+        raise RuntimeError("parasail_hip_nw_banded_batch failed: %s"
+                           % lib().parasail_hip_last_error().decode(errors="replace"))
+    return res

@@ -237,55 +237,3 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
     o.max_val = vmax;
     p.out[pr] = o;
 }
-
-// Banded global alignment, one lane per pair (src/nw_banded.c:29-189).  The
-// band is kept in diagonal coordinates in HBM scratch (slot i of column j
-// is row colOff+i); slots whose row falls outside [0,m) keep their previous
-// value and the two edge slots stay at -inf, as in the reference.
-extern "C" __global__ void __launch_bounds__(64)
-k_banded(const uint8_t *s1, int m, const uint8_t *s2, int n, int open, int gap, int k,
-         const int *mat, int msize, int is_pssm, int *HB, int *EB, phip_out_t *out)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int low = (m > n) ? n - m - k : m - n - k;
-    int colLen = k - low + 1;
-    int colOff = (m > n) ? -k : low;
-    int *H = HB + 1, *E = EB + 1;
-    for (int i = -1; i <= colLen; ++i) { H[i] = 0; E[i] = 0; }
-    for (int i = -colOff + 1, j = 0; i < colLen; ++i, ++j) {
-        H[i] = -open - j * gap;
-        E[i] = PHIP_NEG_INF32;
-    }
-    H[-colOff - 1] = -open;
-    E[-colOff - 1] = PHIP_NEG_INF32;
-    H[-colOff] = 0;
-    E[-colOff] = PHIP_NEG_INF32;
-    H[-1] = PHIP_NEG_INF32;
-    E[-1] = PHIP_NEG_INF32;
-    H[colLen] = PHIP_NEG_INF32;
-    E[colLen] = PHIP_NEG_INF32;
-    for (int j = 0; j < n; ++j) {
-        int F = PHIP_NEG_INF32;
-        if (colOff < 0) H[-colOff - 1] = -open - j * gap;
-        const int lo = imax(0, -colOff), hi = imin(colLen, m - colOff);
-        for (int i = lo; i < hi; ++i) {
-            const int pos = colOff + i;
-            const int matval = is_pssm ? mat[(size_t)msize * pos + s2[j]]
-                                       : mat[(size_t)msize * s1[pos] + s2[j]];
-            const int H_dag = H[i] + matval;
-            const int E_opn = H[i + 1] - open, E_ext = E[i + 1] - gap;
-            const int F_opn = H[i - 1] - open, F_ext = F - gap;
-            E[i] = imax(E_opn, E_ext);
-            F = imax(F_opn, F_ext);
-            H[i] = imax(imax(E[i], F), H_dag);
-        }
-        colOff += 1;
-    }
-    phip_out_t o;
-    memset(&o, 0, sizeof(o));
-    o.score = (m > n) ? H[-low] : H[k];
-    o.end_query = m - 1;
-    o.end_ref = n - 1;
-    o.status = PHIP_ST_W32;
-    *out = o;
-}

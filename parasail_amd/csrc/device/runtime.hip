@@ -38,6 +38,12 @@ extern "C" void phip_set_error(const char *fmt, ...)
 
 extern "C" const char *phip_last_error(void) { return g_err; }
 
+hipEvent_t phip_event(int i) { return i ? g_ev1 : g_ev0; }
+void phip_note_kernel_time()
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_ev0, g_ev1) == hipSuccess) g_last_ms = g_last_main_ms = ms;
+}
 void phip_lock() { pthread_mutex_lock(&g_lock); }
 void phip_unlock() { pthread_mutex_unlock(&g_lock); }
 hipStream_t phip_stream() { return g_stream; }
@@ -363,59 +369,28 @@ extern "C" void phip_resident_free(void *handle)
     pthread_mutex_unlock(&g_lock);
 }
 
-// ---------------------------------------------------------------- banded
-
-extern "C" __global__ void k_banded(const uint8_t *s1, int m, const uint8_t *s2, int n, int open,
-                                    int gap, int k, const int *mat, int msize, int is_pssm,
-                                    int *HB, int *EB, phip_out_t *out);
-
-static int banded_locked(const uint8_t *s1, int m, const uint8_t *s2, int n, int open, int gap,
-                         int k, const phip_matrix_t *mx, phip_out_t *out)
-{
-    const int low = (m > n) ? n - m - k : m - n - k;
-    const size_t colLen = (size_t)(k - low + 1);
-    const size_t mbytes = (size_t)mx->size * mx->length * sizeof(int);
-    size_t off = 0;
-    auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes); return o; };
-    size_t o_mat = slot(mbytes), o_1 = slot(m), o_2 = slot(n), o_h = slot((colLen + 2) * 4),
-           o_e = slot((colLen + 2) * 4), o_out = slot(sizeof(phip_out_t));
-    char *base = (char *)dev_alloc(off);
-    if (!base) return -1;
-    int rc = -1;
-    do {
-        HIP_OK(hipMemcpyAsync(base + o_mat, mx->values, mbytes, hipMemcpyHostToDevice, g_stream));
-        HIP_OK(hipMemcpyAsync(base + o_1, s1, m, hipMemcpyHostToDevice, g_stream));
-        HIP_OK(hipMemcpyAsync(base + o_2, s2, n, hipMemcpyHostToDevice, g_stream));
-        HIP_OK(hipEventRecord(g_ev0, g_stream));
-        hipLaunchKernelGGL(k_banded, dim3(1), dim3(64), 0, g_stream, (const uint8_t *)(base + o_1),
-                           m, (const uint8_t *)(base + o_2), n, open, gap, k,
-                           (const int *)(base + o_mat), mx->size, mx->is_pssm,
-                           (int *)(base + o_h), (int *)(base + o_e), (phip_out_t *)(base + o_out));
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipEventRecord(g_ev1, g_stream));
-        HIP_OK(hipMemcpyAsync(out, base + o_out, sizeof(phip_out_t), hipMemcpyDeviceToHost, g_stream));
-        HIP_OK(hipStreamSynchronize(g_stream));
-        float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
-        g_last_ms = g_last_main_ms = ms;
-        rc = 0;
-    } while (0);
-    dev_free(base);
-    return rc;
-}
-
 extern "C" int phip_nw_banded(const uint8_t *s1, int m, const uint8_t *s2, int n, int open, int gap,
                               int k, const phip_matrix_t *mx, phip_out_t *out)
 {
-    if (phip_init() != 0) return -1;
     if (!s1 || !s2 || m <= 0 || n <= 0 || k <= 0 || !mx || !out) {
         phip_set_error("phip_nw_banded: bad arguments");
         return -1;
     }
-    pthread_mutex_lock(&g_lock);
-    int rc = banded_locked(s1, m, s2, n, open, gap, k, mx, out);
-    pthread_mutex_unlock(&g_lock);
-    return rc;
+    const int64_t zero = 0;
+    phip_batch_t b;
+    memset(&b, 0, sizeof(b));
+    b.alg = PHIP_NW;
+    b.open = open;
+    b.gap = gap;
+    b.matrix = *mx;
+    b.npairs = 1;
+    b.q = s1;
+    b.qoff = &zero;
+    b.qlen = &m;
+    b.t = s2;
+    b.toff = &zero;
+    b.tlen = &n;
+    return phip_nw_banded_batch(&b, k, out);
 }
 
 extern "C" int phip_device_info(int *cus, int *clock_khz)

@@ -8,3 +8,7 @@
 void phip_lock();
 void phip_unlock();
 hipStream_t phip_stream();
+// Events of the shared stream bracketing a call's kernels (0: start,
+// 1: end), and recording their elapsed time as the last call's kernel time.
+hipEvent_t phip_event(int i);
+void phip_note_kernel_time();

@@ -324,3 +324,100 @@ PHOST_EXPORT void parasail_hip_batch_free(parasail_hip_batch_t *hb)
     free(hb->res);
     free(hb);
 }
+
+/* ---- banded global alignment of many pairs ---- */
+
+PHOST_EXPORT int parasail_hip_nw_banded_batch(const char *s1, const int64_t *s1_off,
+                                              const int *s1_len, const char *s2,
+                                              const int64_t *s2_off, const int *s2_len,
+                                              int npairs, int open, int gap, int k,
+                                              const parasail_matrix_t *matrix,
+                                              parasail_hip_batch_result_t *out)
+{
+    if (!s1 || !s1_off || !s1_len || !s2 || !s2_off || !s2_len || npairs < 0 || open < 0 ||
+        gap < 0 || k <= 0 || !matrix || !out) {
+        fprintf(stderr, "%s: invalid arguments\n", __func__);
+        return -1;
+    }
+    if (phip_init() != 0) {
+        errno = ENOSYS;
+        return -1;
+    }
+    if (npairs == 0) return 0;
+    const int pssm = matrix->type == PARASAIL_MATRIX_TYPE_PSSM;
+    int64_t qtot = 0, ttot = 0;
+    int32_t *ql = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
+    int32_t *tl = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
+    int64_t *qo = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
+    int64_t *to = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
+    phip_out_t *res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)npairs);
+    uint8_t *q = NULL, *t = NULL;
+    int rc = -1;
+    if (!ql || !tl || !qo || !to || !res) goto done;
+    for (int p = 0; p < npairs; ++p) {
+        ql[p] = s1_len[p];
+        tl[p] = s2_len[p];
+        if (ql[p] <= 0 || tl[p] <= 0) {
+            fprintf(stderr, "%s: pair %d has an empty sequence\n", __func__, p);
+            goto done;
+        }
+        if (pssm && ql[p] != matrix->length) {
+            fprintf(stderr, "%s: pair %d: query length differs from the PSSM's\n", __func__, p);
+            goto done;
+        }
+        qo[p] = qtot;
+        to[p] = ttot;
+        qtot += ql[p];
+        ttot += tl[p];
+    }
+    q = (uint8_t *)malloc((size_t)qtot + 1);
+    t = (uint8_t *)malloc((size_t)ttot + 1);
+    if (!q || !t) goto done;
+    for (int p = 0; p < npairs; ++p) {
+        const unsigned char *a = (const unsigned char *)s1 + s1_off[p];
+        const unsigned char *b = (const unsigned char *)s2 + s2_off[p];
+        for (int i = 0; i < ql[p]; ++i) q[qo[p] + i] = (uint8_t)matrix->mapper[a[i]];
+        for (int j = 0; j < tl[p]; ++j) t[to[p] + j] = (uint8_t)matrix->mapper[b[j]];
+    }
+    {
+        phip_batch_t bt;
+        memset(&bt, 0, sizeof(bt));
+        bt.alg = PHIP_NW;
+        bt.open = open;
+        bt.gap = gap;
+        bt.matrix.values = matrix->matrix;
+        bt.matrix.size = matrix->size;
+        bt.matrix.length = pssm ? matrix->length : matrix->size;
+        bt.matrix.is_pssm = pssm;
+        bt.matrix.max = matrix->max;
+        bt.matrix.min = matrix->min;
+        bt.npairs = npairs;
+        bt.q = q;
+        bt.qoff = qo;
+        bt.qlen = ql;
+        bt.t = t;
+        bt.toff = to;
+        bt.tlen = tl;
+        if (phip_nw_banded_batch(&bt, k, res) != 0) {
+            fprintf(stderr, "%s: %s\n", __func__, phip_last_error());
+            goto done;
+        }
+    }
+    for (int p = 0; p < npairs; ++p) {
+        if (out->score) out->score[p] = res[p].score;
+        if (out->end_query) out->end_query[p] = res[p].end_query;
+        if (out->end_ref) out->end_ref[p] = res[p].end_ref;
+        if (out->matches) out->matches[p] = 0;
+        if (out->similar) out->similar[p] = 0;
+        if (out->length) out->length[p] = 0;
+        /* the flags parasail_nw_banded_hip sets (src/nw_banded.c:75-77 with
+         * the hip family's width and lanes) */
+        if (out->flag)
+            out->flag[p] = PARASAIL_FLAG_NW | PARASAIL_FLAG_BANDED | PARASAIL_FLAG_BITS_32 |
+                           PARASAIL_FLAG_LANES_64;
+    }
+    rc = 0;
+done:
+    free(ql); free(tl); free(qo); free(to); free(res); free(q); free(t);
+    return rc;
+}

@@ -86,6 +86,9 @@ int phip_nw_banded(const uint8_t *s1, int m, const uint8_t *s2, int n,
                    int open, int gap, int k, const phip_matrix_t *mx,
                    phip_out_t *out);
 
+/* banded global alignment of every pair of b (alg/kind ignored), band k */
+int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *out);
+
 int phip_device_info(int *cus, int *clock_khz);
 
 /* ---- database search and serving (search.hip) ---- */

@@ -360,3 +360,28 @@ def test_config1_full_size_properties(mats):
         bad = np.nonzero((got != exp[:, :3]).any(axis=1))[0]
         assert bad.size == 0, (name, [(int(idx[k]), got[k].tolist(), exp[k, :3].tolist())
                                       for k in bad[:5]])
+
+
+def test_banded_batch(mats):
+    """Many pairs in one launch: bands of every width bucket (2..32 slots per
+    lane), |m - n| beyond the band, a band wider than 64 x 32 slots (the
+    single-lane loop) and pairs of ~20,000 bp with k = 256."""
+    pm, om = mats["dna"]
+    prng = random.Random(16)
+    qs, ts, ks = [], [], []
+    for i in range(48):
+        m = prng.choice([1, 2, 5, 40, 200, 700, 1500, 3000])
+        s1 = rseq(prng, m, "ACGT")
+        s2 = mutate(prng, s1, "ACGT", sub=0.1, indel=0.04) if i % 3 else rseq(prng, prng.randint(1, 3000), "ACGT")
+        qs.append(s1)
+        ts.append(s2)
+    for k in (1, 3, 16, 256):
+        got = P.nw_banded_batch(qs, ts, 5, 2, k, pm)
+        for i in range(len(qs)):
+            exp = O.nw_banded(qs[i], ts[i], 5, 2, k, om)
+            assert (got["score"][i], got["end_query"][i], got["end_ref"][i]) == exp[:3], (k, i)
+    big_q = [rseq(prng, 20000 + 997 * i, "ACGT") for i in range(4)]
+    big_t = [mutate(prng, q, "ACGT", sub=0.1, indel=0.02) for q in big_q]
+    got = P.nw_banded_batch(big_q, big_t, 5, 2, 256, pm)
+    for i in range(4):
+        assert got["score"][i] == O.nw_banded(big_q[i], big_t[i], 5, 2, 256, om)[0], i
