@@ -33,3 +33,10 @@ bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int g
 // of the last wave_plan_run, once that stream has been synchronised; 0 if
 // the run launched none.
 double wave_last_kernel_ms();
+
+// Long-pair global alignment (nwlong.hip): whether a pair goes there, the
+// device scratch it needs, and the launch over a device list of pairs.
+bool nwlong_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int m, int n);
+size_t nwlong_scratch_bytes(int npairs, int maxn);
+int nwlong_launch(const DPParams &dp, const int32_t *dpairs, int npairs, int maxn, char *scratch,
+                  hipStream_t st);

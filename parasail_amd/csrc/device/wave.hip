@@ -556,6 +556,9 @@ struct WavePlan {
     PairRef *drefs = nullptr;
     int32_t *dslow = nullptr, *dredo = nullptr;
     int *dcount = nullptr, *dscratch = nullptr;
+    int nlong = 0;               // long global pairs (nwlong.hip)
+    int32_t *dlong = nullptr;
+    char *dlscr = nullptr;
 };
 
 bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap)
@@ -589,12 +592,15 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     const int tsp0 = (mx.size + 2) & ~1;
     const bool table_ok = (size_t)(mx.size + 2) * (tsp0 / 2) * 4 <= 65535u;
     double total = 0;
+    std::vector<int> lng;
     for (int i = 0; i < np; ++i) {
         P->maxn = std::max(P->maxn, (int)hb.tlen[i]);
         if (table_ok && int16_fits(mx, dp.alg, dp.open, dp.gap, hb.qlen[i], hb.tlen[i])) {
             fast.push_back(i);
             shape[i] = choose(16, hb.qlen[i], hb.tlen[i]);
             total += shape[i].cost;
+        } else if (nwlong_eligible(mx, dp.alg, dp.kind, dp.check, hb.qlen[i], hb.tlen[i])) {
+            lng.push_back(i);
         } else {
             slow.push_back(i);
         }
@@ -756,6 +762,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->nwaves = (int)qw.size();
     P->nfast = (int)refs.size();
     P->nslow = (int)slow.size();
+    P->nlong = (int)lng.size();
     P->nblocks = std::min((P->nwaves + 3) / 4, 1024);
     // DP row 0 as H-open per column (both halves), F = -inf
     const int ntop = maxw + 64 + 16;
@@ -777,7 +784,8 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
            o_top = slot((size_t)ntop * 8), o_dump = slot((size_t)1024 * 256 * 8),
            o_cd = slot((size_t)cdoff * 16), o_refs = slot(refs.size() * sizeof(PairRef)),
            o_slow = slot(slow.size() * 4), o_redo = slot(refs.size() * 4), o_count = slot(64),
-           o_scr = slot(per_lane * P->simple_lanes);
+           o_scr = slot(per_lane * P->simple_lanes), o_long = slot(lng.size() * 4),
+           o_lscr = slot(lng.empty() ? 0 : nwlong_scratch_bytes((int)lng.size(), P->maxn));
     P->base = (char *)dev_alloc(off);
     if (!P->base) { delete P; return nullptr; }
     char *b = P->base;
@@ -794,6 +802,8 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->dredo = (int32_t *)(b + o_redo);
     P->dcount = (int *)(b + o_count);
     P->dscratch = (int *)(b + o_scr);
+    P->dlong = (int32_t *)(b + o_long);
+    P->dlscr = b + o_lscr;
     int *dclen = (int *)(b + o_clen), *dcT = (int *)(b + o_cT);
     auto up = [&](void *dst, const void *src, size_t bytes) -> int {
         if (!bytes) return 0;
@@ -809,6 +819,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     rc |= up(P->dtop, top.data(), (size_t)ntop * 8);
     rc |= up(P->drefs, refs.data(), refs.size() * sizeof(PairRef));
     rc |= up(P->dslow, slow.data(), slow.size() * 4);
+    rc |= up(P->dlong, lng.data(), lng.size() * 4);
     if (nj) {
         hipLaunchKernelGGL(k_colcode, dim3((unsigned)nj), dim3(256), 0, stream, P->djobs, dclen,
                            dcT, (int)nj, dp.t, dp.toff, P->PADT, P->ncopy, P->dcc);
@@ -885,6 +896,7 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
         }
         HIP_OK(hipGetLastError());
     }
+    if (P->nlong && nwlong_launch(dp, P->dlong, P->nlong, P->maxn, P->dlscr, stream) != 0) return -1;
     // int32 tier: pairs outside the int16 window, then SW pairs that overflowed
     const int lanes = P->simple_lanes;
     const dim3 sg((lanes + 255) / 256), sb(256);

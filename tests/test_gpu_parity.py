@@ -385,3 +385,24 @@ def test_banded_batch(mats):
     got = P.nw_banded_batch(big_q, big_t, 5, 2, 256, pm)
     for i in range(4):
         assert got["score"][i] == O.nw_banded(big_q[i], big_t[i], 5, 2, 256, om)[0], i
+
+
+def test_long_nw_int32(mats):
+    """Global scores of pairs too long for int16 (the multi-wave int32
+    kernel): one to three groups of 15 bands, mutated and unrelated pairs,
+    through the batch and the per-call scan entry point."""
+    pm, om = mats["dna"]
+    prng = random.Random(17)
+    shapes = [(20000, 3000), (9000, 9000), (16000, 16100), (40000, 2500), (1500, 20000)]
+    qs, ts = [], []
+    for i, (m, n) in enumerate(shapes):
+        q = rseq(prng, m, "ACGT")
+        t = mutate(prng, q, "ACGT", sub=0.1, indel=0.02)[:n] if i % 2 == 0 else rseq(prng, n, "ACGT")
+        qs.append(q)
+        ts.append(t)
+    got = P.align_batch("nw", 0, 32, qs, ts, 5, 2, pm)
+    exp, _ = O.batch(O.NW, O.SCORE, [om.map(q) for q in qs], [om.map(t) for t in ts], 5, 2, om)
+    for i in range(len(qs)):
+        assert (got["score"][i], got["end_query"][i], got["end_ref"][i]) == tuple(exp[i, :3]), i
+    r = P.align("parasail_nw_scan_hip_64_32", qs[1], ts[1], 5, 2, pm)
+    assert r["score"] == exp[1, 0]
