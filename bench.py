@@ -191,6 +191,51 @@ def serving(nreq, qps):
                                    "gcups": round(300 * nres / alone_s / 1e9, 1)}}
 
 
+def long_pairs(peak32):
+    """Long DNA pairs: 256 pairs of U[20000, 200000] bp, the second a mutated
+    copy of the first (10% substitutions, 2% indels of geometric length,
+    mean 3), match 2 mismatch -3, open 5 extend 2.  Global score in int32
+    (parasail_nw_scan_32 semantics; the multi-wave long-pair kernel) and
+    banded global (parasail_nw_banded, k = 256) of the same pairs, inputs
+    uploaded before the timed kernels."""
+    dm = P.matrix_create("ACGT", 2, -3)
+    t0 = time.perf_counter()
+    b = WL.config3()
+    log("long pairs: %d pairs, %.3e cells (generated in %.1f s)"
+        % (len(b), b.cells, time.perf_counter() - t0))
+    small = b.subset(np.argsort(b.qlen)[:2])
+    P.align_arrays("nw", 0, 32, small, 5, 2, dm)
+    P.nw_banded_arrays(small, 5, 2, 256, dm)
+    t0 = time.perf_counter()
+    r = P.align_arrays("nw", 0, 32, b, 5, 2, dm)
+    wall = time.perf_counter() - t0
+    kms = P.last_kernel_ms()
+    log("long pairs: NW int32 %.1f ms kernel, %.1f ms call" % (kms, wall * 1e3))
+    kb = int(np.count_nonzero(r["flag"] & P.FLAG_BITS_32))
+    t0 = time.perf_counter()
+    rb = P.nw_banded_arrays(b, 5, 2, 256, dm)
+    wall_b = time.perf_counter() - t0
+    kms_b = P.last_kernel_ms()
+    bcells = sum(WL.banded_cells(int(b.qlen[i]), int(b.tlen[i]), 256) for i in range(len(b)))
+    log("long pairs: banded k=256 %.1f ms kernel, %.1f ms call" % (kms_b, wall_b * 1e3))
+    P.lib().parasail_matrix_free(dm)
+    ach = b.cells * 7 / (kms / 1e3)
+    return {"workload": "256 DNA pairs U[20000, 200000] bp, mutated copies (10% sub, 2% indel "
+                        "geom mean 3), +2/-3, open 5 extend 2",
+            "pairs": len(b), "cells": b.cells,
+            "nw_int32": {"gcups": round(b.cells / (kms / 1e3) / 1e9, 1),
+                         "kernel_ms": round(kms, 1), "call_ms": round(wall * 1e3, 1),
+                         "pairs_int32": kb, "kernel": "k_nwlong<16> (15 bands per workgroup)",
+                         "roofline": {"bound": "valu", "achieved": round(ach / 1e12, 2),
+                                      "peak": round(peak32 / 1e12, 2), "unit": "TOP/s",
+                                      "frac": round(ach / peak32, 4),
+                                      "ops_per_cell": 7}},
+            "nw_banded_k256": {"gcups": round(bcells / (kms_b / 1e3) / 1e9, 1), "cells": bcells,
+                               "kernel_ms": round(kms_b, 1), "call_ms": round(wall_b * 1e3, 1),
+                               "score_checksum": int(rb["score"].astype(np.int64).sum())},
+            "nw_score_checksum": int(r["score"].astype(np.int64).sum())}
+
+
 def copy_calibration(mib):
     """One plain device-to-device copy of a known size: under rocprofv3
     --pmc FETCH_SIZE / WRITE_SIZE its counters give the scale from counter
@@ -232,6 +277,7 @@ def main():
                     help="M,N: every pair M x N residues instead of the log-normal mix")
     ap.add_argument("--algs", default="sw,nw,sg")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-long", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--qps", type=float, default=8.0)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -318,6 +364,8 @@ def main():
                                    "workload": "1 x 300 aa query vs 2,000 targets U[50,1000]"}
         if "sw" in per_alg:
             line["batched_over_per_pair_sw"] = round(per_alg["sw"] / g, 1)
+    if not a.no_long:
+        line["long_pairs"] = long_pairs(cus.value * clk.value * 1e3 * 64)
     if not a.no_serve:
         line["serving"] = serving(a.serve_requests, a.qps)
     if not a.no_cpu:

@@ -407,6 +407,19 @@ def nw_banded_batch(queries, targets, open_, gap, k, matrix):
     if n:
         qo[1:] = np.cumsum(ql, dtype=np.int64)[:-1]
         to[1:] = np.cumsum(tl, dtype=np.int64)[:-1]
+    return _nw_banded(b"".join(qb), qo, ql, b"".join(tb), to, tl, open_, gap, k, matrix)
+
+
+def nw_banded_arrays(batch, open_, gap, k, matrix):
+    """parasail_hip_nw_banded_batch over a workloads.PairBatch."""
+    return _nw_banded(batch.q, np.ascontiguousarray(batch.qoff, dtype=np.int64),
+                      np.ascontiguousarray(batch.qlen, dtype=np.int32), batch.t,
+                      np.ascontiguousarray(batch.toff, dtype=np.int64),
+                      np.ascontiguousarray(batch.tlen, dtype=np.int32), open_, gap, k, matrix)
+
+
+def _nw_banded(qcat, qo, ql, tcat, to, tl, open_, gap, k, matrix):
+    n = len(ql)
     res = {k_: np.zeros(n, dtype=np.int32) for k_ in ("score", "end_query", "end_ref", "flag")}
     br = BatchResult()
     for k_, v in res.items():
@@ -417,8 +430,8 @@ def nw_banded_batch(queries, targets, open_, gap, k, matrix):
     L.parasail_hip_nw_banded_batch.argtypes = [ctypes.c_char_p, I64, I32, ctypes.c_char_p, I64, I32,
                                                ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, MP, ctypes.POINTER(BatchResult)]
-    rc = L.parasail_hip_nw_banded_batch(b"".join(qb), qo.ctypes.data_as(I64), ql.ctypes.data_as(I32),
-                                        b"".join(tb), to.ctypes.data_as(I64), tl.ctypes.data_as(I32),
+    rc = L.parasail_hip_nw_banded_batch(qcat, qo.ctypes.data_as(I64), ql.ctypes.data_as(I32),
+                                        tcat, to.ctypes.data_as(I64), tl.ctypes.data_as(I32),
                                         n, open_, gap, k, matrix, ctypes.byref(br))
     if rc != 0:
         raise RuntimeError("parasail_hip_nw_banded_batch failed: %s"

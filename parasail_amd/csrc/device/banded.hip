@@ -42,23 +42,27 @@ struct BandCfg {
     int msize, is_pssm;
     int open, gap, k;
     const int32_t *pairs;  // pairs of this launch
+    const int32_t *rsel;   // slots per lane of each (k_banded_chain)
     int npairs;
     phip_out_t *out;
 };
 
-// Band slots per lane: R (colLen <= 64 R).
-template <int R>
-__global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
+// Slots per lane: R (colLen <= 64 R).  LTAB: the square matrix is staged
+// in LDS; otherwise substitution values come from the matrix or PSSM in
+// global memory (one select of the row index, no branch).
+//
+// Two kinds of step.  Near the ends of the pair some lanes are outside the
+// target, or some slots hold rows outside the query; there every slot is
+// updated only if valid, as in the reference.  In between (every lane on a
+// column of the target, every band slot on a row of the query), all slots
+// are updated unconditionally and the slots past the band's end are capped
+// at -inf; their values only ever reach the band as the left neighbour of
+// its last slot, where any value <= -inf/2 loses the max to a real path just
+// as the reference's -inf does.
+template <int R, bool LTAB>
+__device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, const int pr,
+                                           const int t)
 {
-    extern __shared__ int tab[];
-    const bool ltab = !c.is_pssm && c.msize <= 64;
-    if (ltab)
-        for (int x = threadIdx.x; x < c.msize * c.msize; x += blockDim.x) tab[x] = c.mat[x];
-    __syncthreads();
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= c.npairs) return;  // whole wave
-    const int t = threadIdx.x & 63;
-    const int pr = c.pairs[w];
     const int m = c.qlen[pr], n = c.tlen[pr];
     const uint8_t *s1 = c.q + c.qoff[pr];
     const uint8_t *s2 = c.t + c.toff[pr];
@@ -69,8 +73,9 @@ __global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
     const int tR = t * R;
     const int *mrow = c.mat;
 
-    // the reference's initial arrays (src/nw_banded.c:110-138)
-    int Hb[R], Eb[R], qc[R];
+    // the reference's initial arrays (src/nw_banded.c:110-138); qm holds
+    // the query code of each slot's row times the matrix width
+    int Hb[R], Eb[R], qm[R], cap[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = tR + r;
@@ -81,10 +86,11 @@ __global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
         if (i >= colLen) { h = BNEG; e = BNEG; }
         Hb[r] = h;
         Eb[r] = e;
+        cap[r] = i < colLen ? INT32_MAX : BNEG;
         // the lane's codes shift one row per step from step 0 on, when its
         // column is -t: start from that column's rows
         const int pos = colOff0 - t + i;
-        qc[r] = s1[pos < 0 ? 0 : (pos >= m ? m - 1 : pos)];
+        qm[r] = (int)s1[pos < 0 ? 0 : (pos >= m ? m - 1 : pos)] * msize;
     }
     // clamped, unconditional loads: the residue of a row outside the query
     // is never used (its slot is skipped)
@@ -96,14 +102,15 @@ __global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
     int tq0 = ldt(-t), tq1 = ldt(1 - t);
     int nq0 = ldq(colOff0 - t + tR + R), nq1 = ldq(colOff0 + 1 - t + tR + R);
 
-    auto step = [&](const int s, int &tq, int &nq) __attribute__((always_inline)) {
+    auto step = [&](auto fastc, const int s, int &tq, int &nq) __attribute__((always_inline)) {
+        constexpr bool FAST = decltype(fastc)::value;
         const int j = s - t;
         const bool act = j >= 0 && j < n;
         const int xH = shr1(botH), xF = shr1(botF);
         int Hup = t == 0 ? BNEG : xH;  // slot -1 of lane 0 is the -inf edge
         int F = t == 0 ? BNEG : xF;
         const int colOff = colOff0 + j;
-        if (colOff0 + s - 63 < 0) {
+        if (!FAST && colOff0 + s - 63 < 0) {
             // some lane's column still has rows above the query: the slot
             // just above row 0 holds that column's top boundary
             const int ib = -colOff - 1 - tR;
@@ -118,12 +125,12 @@ __global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int pos = colOff + tR + r;
-            val[r] = act && pos >= 0 && pos < m && tR + r < colLen;
-            if (c.is_pssm) {
-                const int pc = pos < 0 ? 0 : (pos >= m ? m - 1 : pos);
-                W[r] = mrow[(size_t)msize * pc + tc];
+            val[r] = act & (pos >= 0) & (pos < m) & (tR + r < colLen);
+            if (LTAB) {
+                W[r] = tab[qm[r] + tc];
             } else {
-                W[r] = ltab ? tab[qc[r] * msize + tc] : mrow[qc[r] * msize + tc];
+                const int pc = pos < 0 ? 0 : (pos >= m ? m - 1 : pos);
+                W[r] = mrow[(size_t)(c.is_pssm ? pc * msize : qm[r]) + tc];
             }
         }
         auto slot = [&](int r, int Hn, int En) __attribute__((always_inline)) {
@@ -131,10 +138,16 @@ __global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
             const int E = max(Hn - open, En - gap);
             const int Fn = max(Hup - open, F - gap);
             const int H = max(max(E, Fn), Hdag);
-            const bool v = val[r];
-            Eb[r] = v ? E : Eb[r];
-            F = v ? Fn : F;
-            Hb[r] = v ? H : Hb[r];
+            if (FAST) {
+                Eb[r] = E;
+                F = Fn;
+                Hb[r] = min(H, cap[r]);
+            } else {
+                const bool v = val[r];
+                Eb[r] = v ? E : Eb[r];
+                F = v ? Fn : F;
+                Hb[r] = v ? H : Hb[r];
+            }
             Hup = Hb[r];
         };
         slot(0, Hb[1], Eb[1]);
@@ -148,15 +161,31 @@ __global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
         botF = F;
         // next column: every slot moves one row down the query
 #pragma unroll
-        for (int r = 0; r < R - 1; ++r) qc[r] = qc[r + 1];
-        qc[R - 1] = nq;
+        for (int r = 0; r < R - 1; ++r) qm[r] = qm[r + 1];
+        qm[R - 1] = nq * msize;
         tq = ldt(j + 2);
         nq = ldq(colOff + 2 + tR + R);
     };
+    const std::integral_constant<bool, false> slow;
+    const std::integral_constant<bool, true> fast;
+    // steps where every lane is on a column of the target and every band
+    // slot on a row of the query, rounded inwards to even steps
+    const int tl = (colLen - 1) / R;
+    const int sa = (max(63, -colOff0) + 1) & ~1;
+    const int sb = max(sa, min(n, m + tl + 1 - colOff0 - colLen) & ~1);
     const int steps = n + 63;
-    for (int s = 0; s < steps; s += 2) {
-        step(s, tq0, nq0);
-        step(s + 1, tq1, nq1);
+    int s = 0;
+    for (; s < sa; s += 2) {
+        step(slow, s, tq0, nq0);
+        step(slow, s + 1, tq1, nq1);
+    }
+    for (; s < sb; s += 2) {
+        step(fast, s, tq0, nq0);
+        step(fast, s + 1, tq1, nq1);
+    }
+    for (; s < steps; s += 2) {
+        step(slow, s, tq0, nq0);
+        step(slow, s + 1, tq1, nq1);
     }
     const int sidx = (m > n) ? -low : k;
     int score = 0;
@@ -170,6 +199,30 @@ __global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
         o.end_ref = n - 1;
         o.status = PHIP_ST_W32;
         c.out[pr] = o;
+    }
+}
+
+// One wave per pair; c.rsel gives each pair's slots per lane (at most RMAX).
+template <int RMAX, bool LTAB>
+__global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
+{
+    extern __shared__ int tab[];
+    if (LTAB)
+        for (int x = threadIdx.x; x < c.msize * c.msize; x += blockDim.x) tab[x] = c.mat[x];
+    __syncthreads();
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= c.npairs) return;  // whole wave
+    const int t = threadIdx.x & 63;
+    const int pr = c.pairs[w];
+    switch (c.rsel[w]) {
+    case 2: band_chain<2, LTAB>(c, tab, pr, t); break;
+    case 4: band_chain<4, LTAB>(c, tab, pr, t); break;
+    case 8: band_chain<8, LTAB>(c, tab, pr, t); break;
+    case 12: if (RMAX >= 12) band_chain<(RMAX >= 12 ? 12 : 2), LTAB>(c, tab, pr, t); break;
+    case 16: if (RMAX >= 16) band_chain<(RMAX >= 16 ? 16 : 2), LTAB>(c, tab, pr, t); break;
+    case 24: if (RMAX >= 24) band_chain<(RMAX >= 24 ? 24 : 2), LTAB>(c, tab, pr, t); break;
+    case 32: if (RMAX >= 32) band_chain<(RMAX >= 32 ? 32 : 2), LTAB>(c, tab, pr, t); break;
+    default: break;
     }
 }
 
@@ -230,17 +283,18 @@ namespace {
 
 typedef void (*BFn)(BandCfg);
 
-BFn chain_kernel(int R)
+// instances by the largest slots per lane they hold (registers grow with it)
+template <bool LTAB>
+BFn chain_kernel_t(int rmax)
 {
-    switch (R) {
-    case 2: return k_banded_chain<2>;
-    case 4: return k_banded_chain<4>;
-    case 8: return k_banded_chain<8>;
-    case 12: return k_banded_chain<12>;
-    case 16: return k_banded_chain<16>;
-    case 24: return k_banded_chain<24>;
-    default: return k_banded_chain<32>;
-    }
+    if (rmax <= 8) return k_banded_chain<8, LTAB>;
+    if (rmax <= 16) return k_banded_chain<16, LTAB>;
+    return k_banded_chain<32, LTAB>;
+}
+
+BFn chain_kernel(int rmax, bool ltab)
+{
+    return ltab ? chain_kernel_t<true>(rmax) : chain_kernel_t<false>(rmax);
 }
 
 const int kBandRs[] = {2, 4, 8, 12, 16, 24, 32};
@@ -269,8 +323,9 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
     // lane chains for bands up to 64 x 32 slots (bucketed by slots per
     // lane), the single-lane loop for wider ones; gaps large enough to
     // overflow int32 over a band fall back too
-    std::vector<std::vector<int32_t>> bins(sizeof(kBandRs) / sizeof(kBandRs[0]));
-    std::vector<int32_t> wide;
+    std::vector<int32_t> chain, rsel(np, 0), wide;
+    std::vector<int64_t> work(np, 0);
+    int rmax = 0;
     int64_t qtot = 0, ttot = 0, maxcol = 0;
     for (int i = 0; i < np; ++i) {
         const int m = b->qlen[i], n = b->tlen[i];
@@ -283,19 +338,27 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         const int64_t low = (m > n) ? (int64_t)n - m - k : (int64_t)m - n - k;
         const int64_t colLen = k - low + 1;
         maxcol = std::max(maxcol, colLen);
-        int bin = -1;
-        for (size_t x = 0; x < bins.size(); ++x)
-            if (colLen <= 64 * kBandRs[x]) { bin = (int)x; break; }
-        if (bin < 0 || b->open > (1 << 20) || b->gap > (1 << 20)) wide.push_back(i);
-        else bins[bin].push_back(i);
+        int R = 0;
+        for (int r : kBandRs)
+            if (colLen <= 64 * r) { R = r; break; }
+        if (R == 0 || b->open > (1 << 20) || b->gap > (1 << 20)) {
+            wide.push_back(i);
+        } else {
+            chain.push_back(i);
+            rsel[i] = R;
+            work[i] = (int64_t)(n + 64) * (R + 4);
+            rmax = std::max(rmax, R);
+        }
     }
+    // one launch; the longest chains start first
+    std::stable_sort(chain.begin(), chain.end(), [&](int x, int y) { return work[x] > work[y]; });
     const size_t mbytes = (size_t)b->matrix.size * b->matrix.length * 4;
     const int64_t stride = maxcol + 2;
     size_t off = 0;
     auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
     const size_t o_mat = slot(mbytes), o_q = slot((size_t)qtot), o_t = slot((size_t)ttot),
                  o_qo = slot((size_t)np * 8), o_to = slot((size_t)np * 8), o_ql = slot((size_t)np * 4),
-                 o_tl = slot((size_t)np * 4), o_pairs = slot((size_t)np * 4),
+                 o_tl = slot((size_t)np * 4), o_pairs = slot((size_t)np * 4), o_rsel = slot((size_t)np * 4),
                  o_out = slot((size_t)np * sizeof(phip_out_t)),
                  o_scr = slot(wide.empty() ? 0 : (size_t)wide.size() * 2 * stride * 4);
     phip_lock();
@@ -331,35 +394,31 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         ok &= hipMemcpyAsync((void *)c.toff, b->toff, (size_t)np * 8, hipMemcpyHostToDevice, st) == hipSuccess;
         ok &= hipMemcpyAsync((void *)c.qlen, b->qlen, (size_t)np * 4, hipMemcpyHostToDevice, st) == hipSuccess;
         ok &= hipMemcpyAsync((void *)c.tlen, b->tlen, (size_t)np * 4, hipMemcpyHostToDevice, st) == hipSuccess;
-        std::vector<int32_t> order;
-        for (auto &v : bins) order.insert(order.end(), v.begin(), v.end());
+        int32_t *drsel = (int32_t *)(base + o_rsel);
+        std::vector<int32_t> order(chain), rs;
         order.insert(order.end(), wide.begin(), wide.end());
+        for (int i : chain) rs.push_back(rsel[i]);
         ok &= hipMemcpyAsync(dpairs, order.data(), order.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
+        if (!rs.empty())
+            ok &= hipMemcpyAsync(drsel, rs.data(), rs.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
         if (!ok) {
             phip_set_error("phip_nw_banded_batch: upload failed");
             break;
         }
         BOK(hipEventRecord(phip_event(0), st));
-        size_t at = 0;
-        const size_t lds = (!b->matrix.is_pssm && b->matrix.size <= 64)
-                               ? (size_t)b->matrix.size * b->matrix.size * 4 : 0;
-        for (size_t x = 0; x < bins.size(); ++x) {
-            if (!bins[x].empty()) {
-                BandCfg cc = c;
-                cc.pairs = dpairs + at;
-                cc.npairs = (int)bins[x].size();
-                hipLaunchKernelGGL(chain_kernel(kBandRs[x]), dim3((cc.npairs + 3) / 4), dim3(256), lds, st, cc);
-                if (hipGetLastError() != hipSuccess) { ok = false; break; }
-            }
-            at += bins[x].size();
-        }
-        if (!ok) {
-            phip_set_error("phip_nw_banded_batch: launch failed");
-            break;
+        const bool ltab = !b->matrix.is_pssm && b->matrix.size <= 64;
+        const size_t lds = ltab ? (size_t)b->matrix.size * b->matrix.size * 4 : 0;
+        if (!chain.empty()) {
+            BandCfg cc = c;
+            cc.pairs = dpairs;
+            cc.rsel = drsel;
+            cc.npairs = (int)chain.size();
+            hipLaunchKernelGGL(chain_kernel(rmax, ltab), dim3((cc.npairs + 3) / 4), dim3(256), lds, st, cc);
+            BOK(hipGetLastError());
         }
         if (!wide.empty()) {
             BandCfg cc = c;
-            cc.pairs = dpairs + at;
+            cc.pairs = dpairs + chain.size();
             cc.npairs = (int)wide.size();
             hipLaunchKernelGGL(k_banded_one, dim3(cc.npairs), dim3(64), 0, st, cc, (int *)(base + o_scr), stride);
             BOK(hipGetLastError());

@@ -155,3 +155,56 @@ def config4_queries(seed=5, nreq=64, qps=10.0, pssm_frac=0.1):
             pssm = rng.integers(-2, 3, size=(int(ql[k]), 23)).astype(np.int32)  # noise
         out.append((float(arr[k]), q, pssm))
     return out
+
+
+def mutate_fast(rng, s, sub, indel_rate, mean_len):
+    """s (uint8 ACGT array) with substitutions at rate sub (to a different
+    base) and indels at rate indel_rate per base, insertion or deletion with
+    equal odds, geometric lengths of the given mean."""
+    s = s.copy()
+    subs = rng.random(len(s)) < sub
+    codes = np.searchsorted(_DNA_BYTES, s[subs])
+    s[subs] = _DNA_BYTES[(codes + rng.integers(1, 4, size=int(subs.sum()))) % 4]
+    ev = np.nonzero(rng.random(len(s)) < indel_rate)[0]
+    lens = rng.geometric(1.0 / mean_len, size=len(ev))
+    ins = rng.random(len(ev)) < 0.5
+    out, prev = [], 0
+    for p, ln, isins in zip(ev.tolist(), lens.tolist(), ins.tolist()):
+        if p < prev:
+            continue
+        out.append(s[prev:p])
+        if isins:
+            out.append(_DNA_BYTES[rng.integers(0, 4, size=ln)])
+            prev = p
+        else:
+            prev = p + ln
+    out.append(s[prev:])
+    r = np.concatenate(out) if out else s[:0]
+    return r if len(r) else _DNA_BYTES[:1]
+
+
+def config3(seed=3, npairs=256):
+    """Long DNA pairs: lengths uniform in [20000, 200000], the second a copy
+    of the first with 10% substitutions and 2% indels (geometric lengths,
+    mean 3)."""
+    rng = np.random.default_rng(seed)
+    ql = rng.integers(20000, 200001, size=npairs)
+    qs, ts = [], []
+    for k in range(npairs):
+        q = _DNA_BYTES[rng.integers(0, 4, size=int(ql[k]))]
+        qs.append(q)
+        ts.append(mutate_fast(rng, q, 0.10, 0.02, 3))
+    tl = np.array([len(t) for t in ts])
+    return PairBatch(np.concatenate(qs).tobytes(), ql, np.concatenate(ts).tobytes(), tl)
+
+
+def banded_cells(m, n, k):
+    """Cells parasail_nw_banded computes for an m x n pair with band k: per
+    column, the band's rows that fall inside the query."""
+    low = (n - m - k) if m > n else (m - n - k)
+    col_len = k - low + 1
+    off0 = -k if m > n else low
+    j = np.arange(n, dtype=np.int64)
+    lo = np.maximum(0, off0 + j)
+    hi = np.minimum(m, off0 + j + col_len)
+    return int(np.maximum(0, hi - lo).sum())
