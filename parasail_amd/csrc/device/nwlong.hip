@@ -1,22 +1,33 @@
 // nwlong.hip -- global alignment score (parasail_nw / nw_scan_32 semantics,
-// src/nw.c:29-167) of long pairs in int32, one workgroup per pair.
+// src/nw.c:29-167) of long pairs in int32.
 //
-// The pair's query rows are cut into bands of 64*R rows; wave w of the
-// workgroup works on bands w, w+LW, w+2LW, ...  Within a band the 64 lanes
-// form the wavefront chain of the int16 kernels (lane t holds R rows and
-// computes column s - t at step s; the row above comes from lane t-1 by DPP),
-// here in int32 because long global scores leave int16.  The LW bands of a
-// group run as a pipeline: band w starts L = 2*LK steps after band w-1 and
-// takes the bottom row of band w-1 from an LDS ring; all waves meet at a
-// barrier every LK steps, and every ring value a wave reads was written at
-// least one barrier earlier.  The last band of a group leaves its bottom row
-// in HBM for the first band of the next group.
+// The pair's query rows are cut into bands of 64*R rows and the bands into
+// groups of LW.  A workgroup of LW waves computes one group: wave w works on
+// band w of the group, its 64 lanes forming the wavefront chain of the int16
+// kernels (lane t holds R rows and computes column s - t at step s; the row
+// above comes from lane t-1 by DPP), here in int32 because long global scores
+// leave int16.  The LW bands run as a pipeline: band w starts L = 2*LK steps
+// after band w-1 and takes the bottom row of band w-1 from an LDS ring; all
+// waves meet at a barrier every LK steps, and every ring value a wave reads
+// was written at least one barrier earlier.
+//
+// Groups are work items of a queue that persistent workgroups drain in
+// order: the groups of one pair are consecutive items, so groups of a long
+// pair run on several CUs at once, each LK steps behind the one above.  The
+// last band of a group leaves its bottom row in HBM and publishes how many
+// of its columns are final (a release store after the row's stores); the
+// first band of the next group waits, before each LK steps, until the
+// columns it will read are published (an acquire load).  A workgroup only
+// waits on an item taken from the queue before its own, by a workgroup that
+// is already running, so every wait ends; each wait is also bounded, and a
+// wait that runs out marks the launch failed instead of hanging it.
 #include <string.h>
 
 #include <algorithm>
 #include <vector>
 
 #include "search.h"
+#include "wave.h"
 
 namespace {
 
@@ -26,10 +37,22 @@ constexpr int LL = 2 * LK; // lag between consecutive bands
 constexpr int LRING = 512; // ring slots per band boundary (power of two)
 constexpr int LNEG = -(1 << 29);
 constexpr int LWNEG = -(1 << 24);  // substitution value of the pad rows
+constexpr int kLongR = 16;
+constexpr int kGroupRows = LW * 64 * kLongR;
+// bound on the polls of one wait (each sleeps ~0.2 us); legitimate waits
+// are a few thousand steps of the group above, well under a second
+constexpr int kSpinLimit = 1 << 23;
 
 __device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, false); }
 
 }  // namespace
+
+struct LongItem {
+    int pair;      // index into the batch
+    int g;         // group of bands
+    int64_t gin;   // offset (int2) in gbuf of the row above the group, or -1
+    int64_t gout;  // offset (int2) in gbuf of the group's bottom row, or -1
+};
 
 struct LongCfg {
     const uint8_t *q;
@@ -41,11 +64,11 @@ struct LongCfg {
     const int *mat;     // square matrix, size x size
     int msize;
     int open, gap;
-    const int32_t *pairs;
-    int npairs;
-    int2 *gbuf;         // per workgroup: one row of gstride entries
-    int64_t gstride;
-    int2 *dump;         // per thread
+    const LongItem *items;
+    int nitems;
+    int *prog;          // per item: columns of its bottom row final in gbuf
+    int *ctl;           // [0] queue head, [1] failure flag
+    int2 *gbuf;
     phip_out_t *out;
 };
 
@@ -55,35 +78,63 @@ template <int R>
 __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
 {
     extern __shared__ int lds[];
+    __shared__ int s_item;
     const int msize = c.msize, open = c.open, gap = c.gap;
     const int tabn = (msize + 1) * msize;
     for (int x = threadIdx.x; x < tabn; x += blockDim.x)
         lds[x] = x < msize * msize ? c.mat[x] + open : LWNEG;
     int2 *const ring = (int2 *)(lds + ((tabn + 1) & ~1));
-    __syncthreads();
-    if (blockIdx.x >= (unsigned)c.npairs) return;  // whole block
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
-    const int pr = c.pairs[blockIdx.x];
-    const int m = c.qlen[pr], n = c.tlen[pr];
-    const uint8_t *s1 = c.q + c.qoff[pr];
-    const uint8_t *s2 = c.t + c.toff[pr];
-    int2 *const gb = c.gbuf + (int64_t)blockIdx.x * c.gstride;
-    int2 *const dmp = c.dump + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int BR = 64 * R;
-    const int nb = (m + BR - 1) / BR;
-    const int ngroups = (nb + LW - 1) / LW;
-    const int bm = (m - 1) / BR, rem = (m - 1) % BR, tm = rem / R, rm = rem % R;
     const int2 *ring_in = ring + (w > 0 ? w - 1 : 0) * LRING;
     int2 *ring_out = ring + (w < LW - 1 ? w : 0) * LRING;
-    auto left = [&](int g) -> int { return g < 0 ? -open : -open - g * gap - open; };
-    const int G = n + 63 + LL * (LW - 1);
-    const int nph = (G + LK - 1) / LK;
-    int score = 0;
-    bool have = false;
+    int *const failed = c.ctl + 1;
 
-    for (int g = 0; g < ngroups; ++g) {
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int it = atomicAdd(c.ctl, 1);
+            if (__hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) it = c.nitems;
+            s_item = it;
+        }
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item >= c.nitems) break;  // whole block
+        const LongItem li = c.items[item];
+        const int pr = li.pair, g = li.g;
+        const int m = c.qlen[pr], n = c.tlen[pr];
+        const uint8_t *s1 = c.q + c.qoff[pr];
+        const uint8_t *s2 = c.t + c.toff[pr];
+        const int2 *gin = li.gin >= 0 ? c.gbuf + li.gin : nullptr;
+        int2 *gout = li.gout >= 0 ? c.gbuf + li.gout : nullptr;
+        const int BR = 64 * R;
+        const int nb = (m + BR - 1) / BR;
         const int b = g * LW + w;
         const bool live = b < nb;
+        const int bm = (m - 1) / BR, rem = (m - 1) % BR, tm = rem / R, rm = rem % R;
+        auto left = [&](int r) -> int { return r < 0 ? -open : -open - r * gap - open; };
+        const int G = n + 63 + LL * (LW - 1);
+        const int nph = (G + LK - 1) / LK;
+        int score = 0;
+        bool have = false;
+
+        // band 0 of a later group: before steps [ph*LK, ph*LK+LK), the rows
+        // it reads there and two steps beyond (prefetch) must be final
+        auto wait_rows = [&](int ph) {
+            if (gin && w == 0 && t == 0) {
+                const int need = min(n, ph * LK + LK + 2);
+                const int *pin = c.prog + item - 1;
+                int spins = 0;
+                while (__hip_atomic_load(pin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    if (__hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                        ++spins > kSpinLimit) {
+                        __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(8);
+                }
+            }
+        };
+
         const int first_row = b * BR + t * R;
         int qo[R], HL[R], E[R];
 #pragma unroll
@@ -100,15 +151,17 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
         auto ld_in = [&](int jj) -> int2 {
             const int jc = jj < 0 ? 0 : (jj >= n ? n - 1 : jj);
             if (w == 0) {
-                if (g == 0) return make_int2(-open - jc * gap - open, LNEG);
-                return gb[jc];
+                if (!gin) return make_int2(-open - jc * gap - open, LNEG);
+                return gin[jc];
             }
             return ring_in[jc & (LRING - 1)];
         };
         auto ld_t = [&](int jj) -> int { return (int)s2[jj < 0 ? 0 : (jj >= n ? n - 1 : jj)]; };
         const int s0 = -LL * w;  // this band's step at group step 0
+        wait_rows(0);
         int2 in0 = ld_in(s0), in1 = ld_in(s0 + 1);
         int tc0 = ld_t(s0 - t), tc1 = ld_t(s0 + 1 - t);
+        const bool pub = gout && w == LW - 1 && t == 63;
 
         auto step = [&](const int s, int2 &bx, int &tq) __attribute__((always_inline)) {
             const int j = s - t;
@@ -136,8 +189,7 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
                 botH = HL[R - 1];
                 botF = F;
                 if (t == 63 && w < LW - 1) ring_out[j & (LRING - 1)] = make_int2(botH, botF);
-                int2 *dst = (t == 63 && w == LW - 1) ? gb + j : dmp;
-                *dst = make_int2(botH, botF);
+                if (pub) gout[j] = make_int2(botH, botF);
                 if (b == bm && t == tm && j == n - 1) {
 #pragma unroll
                     for (int i = 0; i < R; ++i)
@@ -155,26 +207,44 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
                     step(sb + k, in0, tc0);
                     step(sb + k + 1, in1, tc1);
                 }
+                if (pub) {
+                    // columns of the bottom row stored so far, then published
+                    const int done = min(n, max(0, (ph + 1) * LK - LL * (LW - 1) - 63));
+                    __hip_atomic_store(c.prog + item, done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
+            if (ph + 1 < nph) wait_rows(ph + 1);
             __syncthreads();
         }
+        if (have) {
+            phip_out_t o;
+            memset(&o, 0, sizeof(o));
+            o.score = score;
+            o.end_query = m - 1;
+            o.end_ref = n - 1;
+            o.status = __hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           ? PHIP_ST_ERR : PHIP_ST_W32;
+            c.out[pr] = o;
+        }
     }
-    if (have) {
-        phip_out_t o;
-        memset(&o, 0, sizeof(o));
-        o.score = score;
-        o.end_query = m - 1;
-        o.end_ref = n - 1;
-        o.status = PHIP_ST_W32;
-        c.out[pr] = o;
-    }
+}
+
+__global__ void k_nwlong_zero(int *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// A failed launch (a wait ran out) leaves every long pair marked.
+__global__ void k_nwlong_check(const int *ctl, const LongItem *items, int nitems, phip_out_t *out)
+{
+    if (!ctl[1]) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x)
+        out[items[i].pair].status = PHIP_ST_ERR;
 }
 
 // ------------------------------------------------------------------ host
 
 namespace {
-
-constexpr int kLongR = 16;
 
 size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -187,15 +257,61 @@ bool nwlong_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int 
            m >= 1024 && n >= 64 && (int64_t)m * n >= ((int64_t)1 << 24);
 }
 
-size_t nwlong_scratch_bytes(int npairs, int maxn)
+bool nwlong_plan(const HostBatch &hb, const std::vector<int> &pairs, LongPlan &lp)
 {
-    return al256((size_t)npairs * ((size_t)maxn + 64) * 8) + al256((size_t)npairs * LW * 64 * 8);
+    lp = LongPlan();
+    if (pairs.empty()) return true;
+    const int fill = LL * (LW - 1) + 64;
+    // longest chain first: a pair's last group ends about n + groups * fill
+    // steps after its first starts
+    std::vector<int> ord(pairs);
+    auto groups = [&](int i) { return (hb.qlen[i] + kGroupRows - 1) / kGroupRows; };
+    auto chain = [&](int i) { return (int64_t)hb.tlen[i] + (int64_t)groups(i) * fill; };
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return chain(x) > chain(y); });
+    std::vector<LongItem> items;
+    int64_t goff = 0;
+    for (int i : ord) {
+        const int ng = groups(i);
+        int64_t prev = -1;
+        for (int g = 0; g < ng; ++g) {
+            LongItem it;
+            it.pair = i;
+            it.g = g;
+            it.gin = prev;
+            it.gout = -1;
+            if (g + 1 < ng) {
+                it.gout = goff;
+                goff += (int64_t)hb.tlen[i] + 64;
+            }
+            prev = it.gout;
+            items.push_back(it);
+        }
+    }
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int tabn = (hb.mx->size + 1) * hb.mx->size;
+    lp.lds = (size_t)((tabn + 1) & ~1) * 4 + (size_t)(LW - 1) * LRING * 8;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nwlong<kLongR>, LW * 64, lp.lds) !=
+            hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    lp.nitems = (int)items.size();
+    lp.grid = std::min(lp.nitems, cus * per_cu);
+    size_t off = 0;
+    auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
+    lp.o_items = slot(items.size() * sizeof(LongItem));
+    lp.o_prog = slot((size_t)lp.nitems * 4 + 64);
+    lp.o_gbuf = slot((size_t)goff * 8);
+    lp.bytes = off;
+    lp.items.resize(items.size() * sizeof(LongItem));
+    memcpy(lp.items.data(), items.data(), lp.items.size());
+    return true;
 }
 
-int nwlong_launch(const DPParams &dp, const int32_t *dpairs, int npairs, int maxn, char *scratch,
-                  hipStream_t st)
+int nwlong_launch(const DPParams &dp, const LongPlan &lp, char *region, hipStream_t st)
 {
-    if (npairs <= 0) return 0;
+    if (lp.nitems <= 0) return 0;
     LongCfg c;
     memset(&c, 0, sizeof(c));
     c.q = dp.q;
@@ -208,15 +324,17 @@ int nwlong_launch(const DPParams &dp, const int32_t *dpairs, int npairs, int max
     c.msize = dp.msize;
     c.open = dp.open;
     c.gap = dp.gap;
-    c.pairs = dpairs;
-    c.npairs = npairs;
-    c.gstride = (int64_t)maxn + 64;
-    c.gbuf = (int2 *)scratch;
-    c.dump = (int2 *)(scratch + al256((size_t)npairs * c.gstride * 8));
+    c.items = (const LongItem *)(region + lp.o_items);
+    c.nitems = lp.nitems;
+    c.prog = (int *)(region + lp.o_prog);
+    c.ctl = c.prog + lp.nitems;
+    c.gbuf = (int2 *)(region + lp.o_gbuf);
     c.out = dp.out;
-    const int tabn = (dp.msize + 1) * dp.msize;
-    const size_t lds = (size_t)((tabn + 1) & ~1) * 4 + (size_t)(LW - 1) * LRING * 8;
-    hipLaunchKernelGGL(k_nwlong<kLongR>, dim3(npairs), dim3(LW * 64), lds, st, c);
+    hipLaunchKernelGGL(k_nwlong_zero, dim3(std::min((lp.nitems + 2 + 255) / 256, 1024)), dim3(256), 0,
+                       st, c.prog, lp.nitems + 2);
+    hipLaunchKernelGGL(k_nwlong<kLongR>, dim3(lp.grid), dim3(LW * 64), lp.lds, st, c);
+    hipLaunchKernelGGL(k_nwlong_check, dim3(std::min((lp.nitems + 255) / 256, 1024)), dim3(256), 0, st,
+                       c.ctl, c.items, lp.nitems, c.out);
     HIP_OK(hipGetLastError());
     return 0;
 }

@@ -307,6 +307,12 @@ static int resident_fetch(phip_resident_t *r, phip_out_t *out, int8_t **traces)
         HIP_OK(hipMemcpyAsync(out, r->dout, (size_t)r->np * sizeof(phip_out_t),
                               hipMemcpyDeviceToHost, g_stream));
     HIP_OK(hipStreamSynchronize(g_stream));
+    if (out)
+        for (int i = 0; i < r->np; ++i)
+            if (out[i].status & PHIP_ST_ERR) {
+                phip_set_error("pair %d could not be computed (long-pair pipeline failed)", i);
+                return -1;
+            }
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
     g_last_ms = ms;

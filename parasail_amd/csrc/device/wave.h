@@ -34,9 +34,16 @@ bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int g
 // the run launched none.
 double wave_last_kernel_ms();
 
-// Long-pair global alignment (nwlong.hip): whether a pair goes there, the
-// device scratch it needs, and the launch over a device list of pairs.
+// Long-pair global alignment (nwlong.hip): whether a pair goes there; the
+// plan (work items of a device queue: groups of bands, the groups of a pair
+// consecutive) and the device region it needs, whose first lp.items.size()
+// bytes the caller uploads from lp.items; the launch.
+struct LongPlan {
+    std::vector<char> items;
+    size_t bytes = 0, lds = 0;
+    size_t o_items = 0, o_prog = 0, o_gbuf = 0;
+    int nitems = 0, grid = 0;
+};
 bool nwlong_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int m, int n);
-size_t nwlong_scratch_bytes(int npairs, int maxn);
-int nwlong_launch(const DPParams &dp, const int32_t *dpairs, int npairs, int maxn, char *scratch,
-                  hipStream_t st);
+bool nwlong_plan(const HostBatch &hb, const std::vector<int> &pairs, LongPlan &lp);
+int nwlong_launch(const DPParams &dp, const LongPlan &lp, char *region, hipStream_t st);

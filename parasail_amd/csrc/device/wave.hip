@@ -556,9 +556,8 @@ struct WavePlan {
     PairRef *drefs = nullptr;
     int32_t *dslow = nullptr, *dredo = nullptr;
     int *dcount = nullptr, *dscratch = nullptr;
-    int nlong = 0;               // long global pairs (nwlong.hip)
-    int32_t *dlong = nullptr;
-    char *dlscr = nullptr;
+    LongPlan lp;                 // long global pairs (nwlong.hip)
+    char *dlong = nullptr;
 };
 
 bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap)
@@ -762,7 +761,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->nwaves = (int)qw.size();
     P->nfast = (int)refs.size();
     P->nslow = (int)slow.size();
-    P->nlong = (int)lng.size();
+    if (!nwlong_plan(hb, lng, P->lp)) { delete P; return nullptr; }
     P->nblocks = std::min((P->nwaves + 3) / 4, 1024);
     // DP row 0 as H-open per column (both halves), F = -inf
     const int ntop = maxw + 64 + 16;
@@ -784,8 +783,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
            o_top = slot((size_t)ntop * 8), o_dump = slot((size_t)1024 * 256 * 8),
            o_cd = slot((size_t)cdoff * 16), o_refs = slot(refs.size() * sizeof(PairRef)),
            o_slow = slot(slow.size() * 4), o_redo = slot(refs.size() * 4), o_count = slot(64),
-           o_scr = slot(per_lane * P->simple_lanes), o_long = slot(lng.size() * 4),
-           o_lscr = slot(lng.empty() ? 0 : nwlong_scratch_bytes((int)lng.size(), P->maxn));
+           o_scr = slot(per_lane * P->simple_lanes), o_long = slot(P->lp.bytes);
     P->base = (char *)dev_alloc(off);
     if (!P->base) { delete P; return nullptr; }
     char *b = P->base;
@@ -802,8 +800,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->dredo = (int32_t *)(b + o_redo);
     P->dcount = (int *)(b + o_count);
     P->dscratch = (int *)(b + o_scr);
-    P->dlong = (int32_t *)(b + o_long);
-    P->dlscr = b + o_lscr;
+    P->dlong = b + o_long;
     int *dclen = (int *)(b + o_clen), *dcT = (int *)(b + o_cT);
     auto up = [&](void *dst, const void *src, size_t bytes) -> int {
         if (!bytes) return 0;
@@ -819,7 +816,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     rc |= up(P->dtop, top.data(), (size_t)ntop * 8);
     rc |= up(P->drefs, refs.data(), refs.size() * sizeof(PairRef));
     rc |= up(P->dslow, slow.data(), slow.size() * 4);
-    rc |= up(P->dlong, lng.data(), lng.size() * 4);
+    rc |= up(P->dlong, P->lp.items.data(), P->lp.items.size());
     if (nj) {
         hipLaunchKernelGGL(k_colcode, dim3((unsigned)nj), dim3(256), 0, stream, P->djobs, dclen,
                            dcT, (int)nj, dp.t, dp.toff, P->PADT, P->ncopy, P->dcc);
@@ -896,7 +893,7 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
         }
         HIP_OK(hipGetLastError());
     }
-    if (P->nlong && nwlong_launch(dp, P->dlong, P->nlong, P->maxn, P->dlscr, stream) != 0) return -1;
+    if (P->lp.nitems && nwlong_launch(dp, P->lp, P->dlong, stream) != 0) return -1;
     // int32 tier: pairs outside the int16 window, then SW pairs that overflowed
     const int lanes = P->simple_lanes;
     const dim3 sg((lanes + 255) / 256), sb(256);
