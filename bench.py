@@ -191,6 +191,47 @@ def serving(nreq, qps):
                                    "gcups": round(300 * nres / alone_s / 1e9, 1)}}
 
 
+def stats_trace(npairs=100000):
+    """DNA reads against their windows: 100,000 pairs of a 250 bp read (4%
+    substitutions, 0.5% insertions and 0.5% deletions of geometric length,
+    mean 2) and its 400 bp window, match 2 mismatch -3, open 5 extend 2;
+    semi-global (all four ends free) with statistics, then with traceback
+    and CIGAR strings, int16-first."""
+    dm = P.matrix_create("ACGT", 2, -3)
+    t0 = time.perf_counter()
+    b = WL.config2(npairs=npairs)
+    log("stats/trace: %d pairs, %.3e cells (generated in %.1f s)"
+        % (len(b), b.cells, time.perf_counter() - t0))
+    small = b.subset(np.arange(min(64, len(b))))
+    P.align_arrays("sg", 1, 0, small, 5, 2, dm)
+    P.align_arrays("sg", 2, 0, small, 5, 2, dm, want_cigar=True)
+    t0 = time.perf_counter()
+    rs = P.align_arrays("sg", 1, 0, b, 5, 2, dm)
+    wall_s = time.perf_counter() - t0
+    kms_s = P.last_kernel_ms()
+    log("stats/trace: stats %.1f ms kernel, %.1f ms call" % (kms_s, wall_s * 1e3))
+    t0 = time.perf_counter()
+    rt = P.align_arrays("sg", 2, 0, b, 5, 2, dm, want_cigar=True)
+    wall_t = time.perf_counter() - t0
+    kms_t = P.last_kernel_ms()
+    log("stats/trace: trace+cigar %.1f ms kernel, %.1f ms call" % (kms_t, wall_t * 1e3))
+    P.lib().parasail_matrix_free(dm)
+    ncig = sum(1 for c in rt["cigar"] if c)
+    return {"workload": "100,000 DNA pairs: 250 bp reads (4% sub, 0.5% ins, 0.5% del, geom "
+                        "mean 2) vs 400 bp windows, +2/-3, open 5 extend 2, sg all ends free",
+            "pairs": len(b), "cells": b.cells,
+            "sg_stats": {"gcups_kernel": round(b.cells / (kms_s / 1e3) / 1e9, 1),
+                         "gcups_call": round(b.cells / wall_s / 1e9, 1),
+                         "kernel_ms": round(kms_s, 1), "call_ms": round(wall_s * 1e3, 1),
+                         "checksum": int(rs["score"].astype(np.int64).sum()
+                                         + rs["matches"].astype(np.int64).sum()
+                                         + rs["length"].astype(np.int64).sum())},
+            "sg_trace_cigar": {"gcups_kernel": round(b.cells / (kms_t / 1e3) / 1e9, 1),
+                               "gcups_call": round(b.cells / wall_t / 1e9, 1),
+                               "kernel_ms": round(kms_t, 1), "call_ms": round(wall_t * 1e3, 1),
+                               "cigars": ncig}}
+
+
 def long_pairs(peak32):
     """Long DNA pairs: 256 pairs of U[20000, 200000] bp, the second a mutated
     copy of the first (10% substitutions, 2% indels of geometric length,
@@ -278,6 +319,7 @@ def main():
     ap.add_argument("--algs", default="sw,nw,sg")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-long", action="store_true")
+    ap.add_argument("--no-stats", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--qps", type=float, default=8.0)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -364,6 +406,8 @@ def main():
                                    "workload": "1 x 300 aa query vs 2,000 targets U[50,1000]"}
         if "sw" in per_alg:
             line["batched_over_per_pair_sw"] = round(per_alg["sw"] / g, 1)
+    if not a.no_stats:
+        line["stats_trace"] = stats_trace()
     if not a.no_long:
         line["long_pairs"] = long_pairs(cus.value * clk.value * 1e3 * 64)
     if not a.no_serve:

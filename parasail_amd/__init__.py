@@ -214,16 +214,20 @@ def cigar_tuple(c):
     return out
 
 
-def align_arrays(alg, kind, width, batch, open_, gap, matrix):
+def align_arrays(alg, kind, width, batch, open_, gap, matrix, want_cigar=False):
     """Batched alignment of a workloads.PairBatch (concatenated residues with
-    offsets and lengths) through parasail_hip_align_batch; score-only or
-    stats results as numpy arrays."""
+    offsets and lengths) through parasail_hip_align_batch; results as numpy
+    arrays (and, for traceback with want_cigar, a list of CIGAR tuples)."""
     n = len(batch)
     res = {k: np.zeros(n, dtype=np.int32) for k in
            ("score", "end_query", "end_ref", "matches", "similar", "length", "flag")}
     br = BatchResult()
     for k in res:
         setattr(br, k, res[k].ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    cig = None
+    if want_cigar:
+        cig = (ctypes.POINTER(Cigar) * max(n, 1))()
+        br.cigars = ctypes.cast(cig, ctypes.POINTER(ctypes.POINTER(Cigar)))
     I64 = ctypes.POINTER(ctypes.c_int64)
     I32 = ctypes.POINTER(ctypes.c_int)
     qoff = np.ascontiguousarray(batch.qoff, dtype=np.int64)
@@ -237,6 +241,8 @@ def align_arrays(alg, kind, width, batch, open_, gap, matrix):
     if rc != 0:
         raise RuntimeError("parasail_hip_align_batch failed: %s"
                            % lib().parasail_hip_last_error().decode(errors="replace"))
+    if want_cigar:
+        res["cigar"] = [cigar_tuple(cig[k]) for k in range(n)]
     return res
 
 

@@ -1,0 +1,49 @@
+// dp32.h -- int32 wavefront kernels for _stats and _trace (dp32.hip) and
+// the device CIGAR walk.
+#pragma once
+
+#include <vector>
+
+#include "wave.h"
+
+// Whether a pair of this length and kind goes to the wavefront kernel.
+// check: 0, or the width whose saturation the caller tests; the kernel
+// returns no extrema, so it takes 16 only where the 16-bit window provably
+// holds every value.
+bool dp32_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap, int m,
+                   int n);
+// Trace layout code of the wavefront kernel for a query of m rows
+// (T | R << 8; 0 if the pair is outside its range) and the bytes its trace
+// takes for n columns.
+int32_t dp32_layout(int m);
+int64_t dp32_trace_bytes(int m, int n);
+
+// Jobs and waves of one pass, as a host image of the device region the
+// caller uploads (bytes long).
+struct Dp32Plan {
+    std::vector<char> image;
+    size_t bytes = 0, o_jobs = 0, o_waves = 0;
+    int njobs = 0, nwaves = 0;
+};
+// status[p] and troff[p] (may be NULL) are indexed by pair.
+bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t *status,
+               const int64_t *troff, Dp32Plan &P);
+int dp32_launch(const DPParams &dp, const Dp32Plan &P, const char *region, hipStream_t st);
+
+// CIGAR of every pair from its trace (row-major when tlay[p] is 0, else
+// the wavefront order of layout tlay[p]), ends from out[]; ops[opoff[p]..]
+// receives nops[p] ops (-1: none), beg[2p], beg[2p+1] the start.
+struct CigarArgs {
+    const uint8_t *qraw, *traw;
+    const int64_t *qoff, *toff;
+    const int32_t *qlen, *tlen;
+    const int8_t *trace;
+    const int64_t *troff;
+    const int32_t *tlay;
+    const phip_out_t *out;
+    int is_sg, npairs;
+    uint32_t *ops;
+    const int64_t *opoff;
+    int32_t *nops, *beg;
+};
+int cigar_launch(const CigarArgs &a, hipStream_t st);

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "common.h"
+#include "dp32.h"
 #include "wave.h"
 
 static char g_err[512];
@@ -145,6 +146,14 @@ struct phip_resident {
     int64_t trtot;
     std::vector<WavePlan *> plans;
     bool last_wave = false;  // the last pass ran the wavefront tier
+    // _stats / _trace: the wavefront plan (built for one check, alg, open,
+    // gap), its device image, and per pair the trace layout of the last
+    // pass (0 row-major, else dp32_layout)
+    Dp32Plan dplan;
+    int dplan_key[4] = {-1, -1, -1, -1};
+    char *dplan_mem = nullptr;
+    std::vector<int32_t> slow, tlay;
+    int32_t *dslow = nullptr, *dtlay = nullptr;
 };
 
 typedef struct phip_resident phip_resident_t;
@@ -172,8 +181,10 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
         ttot = std::max<int64_t>(ttot, b->toff[i] + b->tlen[i]);
         r->maxn = std::max(r->maxn, (int)b->tlen[i]);
         if (b->kind == PHIP_TRACE) {
+            // room for either layout of the pair's trace
             r->troff[i] = r->trtot;
-            r->trtot += (int64_t)b->qlen[i] * b->tlen[i];
+            const int64_t rm = (int64_t)b->qlen[i] * b->tlen[i];
+            r->trtot += (std::max(rm, dp32_trace_bytes(b->qlen[i], b->tlen[i])) + 15) & ~(int64_t)15;
         }
     }
     size_t off = 0;
@@ -181,7 +192,7 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
     size_t o_mat = slot(r->mvals.size() * 4), o_q = slot(r->has_q ? qtot : 0), o_t = slot(ttot),
            o_qo = slot(np * 8), o_to = slot(np * 8), o_ql = slot(np * 4), o_tl = slot(np * 4),
            o_out = slot(np * sizeof(phip_out_t)), o_tro = slot(r->troff.size() * 8),
-           o_tr = slot(r->trtot);
+           o_tr = slot(r->trtot), o_slow = slot(np * 4), o_tlay = slot(np * 4);
     r->base = (char *)dev_alloc(off);
     if (!r->base) { delete r; return nullptr; }
     r->dmat = (int *)(r->base + o_mat);
@@ -194,6 +205,8 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
     r->dout = (phip_out_t *)(r->base + o_out);
     r->dtroff = (int64_t *)(r->base + o_tro);
     r->dtr = b->kind == PHIP_TRACE ? (int8_t *)(r->base + o_tr) : nullptr;
+    r->dslow = (int32_t *)(r->base + o_slow);
+    r->dtlay = (int32_t *)(r->base + o_tlay);
     bool ok = true;
     ok &= hipMemcpyAsync(r->dmat, r->mvals.data(), r->mvals.size() * 4, hipMemcpyHostToDevice, g_stream) == hipSuccess;
     if (r->has_q) ok &= hipMemcpyAsync(r->dq, b->q, qtot, hipMemcpyHostToDevice, g_stream) == hipSuccess;
@@ -218,6 +231,7 @@ static void resident_free(phip_resident_t *r)
 {
     if (!r) return;
     for (WavePlan *p : r->plans) wave_plan_free(p);
+    if (r->dplan_mem) dev_free(r->dplan_mem);
     dev_free(r->base);
     delete r;
 }
@@ -249,6 +263,8 @@ static DPParams make_params(const phip_resident_t *r, int alg, int kind, int sg_
     return dp;
 }
 
+static int simple_run(phip_resident_t *r, const DPParams &dp, const int32_t *list, int count);
+
 // Runs one alignment pass over the resident batch; results stay in r->dout.
 static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int open, int gap,
                         int check)
@@ -275,29 +291,91 @@ static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int
         HIP_OK(hipEventRecord(g_ev0, g_stream));
         if (wave_plan_run(plan, dp, g_stream) != 0) return -1;
     } else {
-        const int narr = kind == PHIP_STATS ? 8 : 2;
-        const size_t per_lane = (size_t)(r->maxn + 1) * narr * sizeof(int);
-        const int lanes = (int)std::max<size_t>(
-            64, std::min<size_t>((size_t)r->np, ((size_t)1 << 30) / per_lane));
-        int *scratch = (int *)dev_alloc(per_lane * lanes);
-        if (!scratch) return -1;
-        HIP_OK(hipEventRecord(g_ev0, g_stream));
-        for (int first = 0; first < r->np; first += lanes) {
-            const int count = std::min(lanes, r->np - first);
-            hipLaunchKernelGGL(k_simple, dim3((count + 255) / 256), dim3(256), 0, g_stream, dp,
-                               first, count, (const int32_t *)nullptr, (const int *)nullptr,
-                               scratch, count, r->maxn);
-            if (hipGetLastError() != hipSuccess) {
-                dev_free(scratch);
-                phip_set_error("k_simple launch failed");
+        // the int32 wavefront kernel for the pairs it covers, the general
+        // tier for the rest
+        const int key[4] = {check, alg, open, gap};
+        if (memcmp(key, r->dplan_key, sizeof(key)) != 0) {
+            HostBatch hb;
+            hb.np = r->np;
+            hb.qlen = r->qlen.data();
+            hb.tlen = r->tlen.data();
+            hb.qoff = r->qoff.data();
+            hb.toff = r->toff.data();
+            hb.mx = &r->mx;
+            std::vector<int> fast;
+            std::vector<int32_t> status(r->np, PHIP_ST_W32);
+            r->slow.clear();
+            r->tlay.assign(r->np, 0);
+            for (int i = 0; i < r->np; ++i) {
+                if (dp32_eligible(r->mx, alg, kind, check, open, gap, r->qlen[i], r->tlen[i])) {
+                    fast.push_back(i);
+                    r->tlay[i] = dp32_layout(r->qlen[i]);
+                } else {
+                    r->slow.push_back(i);
+                }
+            }
+            if (!dp32_plan(hb, fast, status.data(), r->troff.empty() ? nullptr : r->troff.data(),
+                           r->dplan)) {
+                phip_set_error("dp32 plan failed");
                 return -1;
             }
+            if (r->dplan_mem) dev_free(r->dplan_mem);
+            r->dplan_mem = nullptr;
+            if (r->dplan.bytes) {
+                r->dplan_mem = (char *)dev_alloc(r->dplan.bytes);
+                if (!r->dplan_mem) return -1;
+                HIP_OK(hipMemcpyAsync(r->dplan_mem, r->dplan.image.data(), r->dplan.bytes,
+                                      hipMemcpyHostToDevice, g_stream));
+            }
+            if (!r->slow.empty())
+                HIP_OK(hipMemcpyAsync(r->dslow, r->slow.data(), r->slow.size() * 4,
+                                      hipMemcpyHostToDevice, g_stream));
+            HIP_OK(hipMemcpyAsync(r->dtlay, r->tlay.data(), (size_t)r->np * 4,
+                                  hipMemcpyHostToDevice, g_stream));
+            // the pageable host vectors stay alive in r; the copies must end
+            // before they change
+            HIP_OK(hipStreamSynchronize(g_stream));
+            memcpy(r->dplan_key, key, sizeof(key));
         }
-        // the block goes back to the pool; later users queue behind these
-        // kernels on the same stream
-        dev_free(scratch);
+        HIP_OK(hipEventRecord(g_ev0, g_stream));
+        if (dp32_launch(dp, r->dplan, r->dplan_mem, g_stream) != 0) return -1;
+        if (!r->slow.empty() && simple_run(r, dp, r->dslow, (int)r->slow.size()) != 0) return -1;
     }
     HIP_OK(hipEventRecord(g_ev1, g_stream));
+    return 0;
+}
+
+// The general int32 tier over `count` pairs: all of the batch (list NULL)
+// or the device list.
+static int simple_run(phip_resident_t *r, const DPParams &dp, const int32_t *list, int count)
+{
+    const int narr = dp.kind == PHIP_STATS ? 8 : 2;
+    const size_t per_lane = (size_t)(r->maxn + 1) * narr * sizeof(int);
+    const int lanes = (int)std::max<size_t>(
+        64, std::min<size_t>((size_t)count, ((size_t)1 << 30) / per_lane));
+    int *scratch = (int *)dev_alloc(per_lane * lanes);
+    if (!scratch) return -1;
+    bool ok = true;
+    if (list) {
+        hipLaunchKernelGGL(k_simple, dim3((lanes + 255) / 256), dim3(256), 0, g_stream, dp, 0, count,
+                           list, (const int *)nullptr, scratch, lanes, r->maxn);
+        ok = hipGetLastError() == hipSuccess;
+    } else {
+        for (int first = 0; ok && first < count; first += lanes) {
+            const int cnt = std::min(lanes, count - first);
+            hipLaunchKernelGGL(k_simple, dim3((cnt + 255) / 256), dim3(256), 0, g_stream, dp, first,
+                               cnt, (const int32_t *)nullptr, (const int *)nullptr, scratch, cnt,
+                               r->maxn);
+            ok = hipGetLastError() == hipSuccess;
+        }
+    }
+    // the block goes back to the pool; later users queue behind these
+    // kernels on the same stream
+    dev_free(scratch);
+    if (!ok) {
+        phip_set_error("k_simple launch failed");
+        return -1;
+    }
     return 0;
 }
 
@@ -318,10 +396,26 @@ static int resident_fetch(phip_resident_t *r, phip_out_t *out, int8_t **traces)
     g_last_ms = ms;
     g_last_main_ms = r->last_wave ? wave_last_kernel_ms() : ms;
     if (traces && r->dtr) {
+        std::vector<int8_t> wbuf;
         for (int i = 0; i < r->np; ++i) {
             if (!traces[i]) continue;
-            HIP_OK(hipMemcpy(traces[i], r->dtr + r->troff[i], (size_t)r->qlen[i] * r->tlen[i],
-                             hipMemcpyDeviceToHost));
+            const int m = r->qlen[i], n = r->tlen[i];
+            const int lay = r->tlay.empty() ? 0 : r->tlay[i];
+            if (!lay) {
+                HIP_OK(hipMemcpy(traces[i], r->dtr + r->troff[i], (size_t)m * n,
+                                 hipMemcpyDeviceToHost));
+                continue;
+            }
+            // wavefront order (row t*R+r, column j at ((j+t)*T + t)*R + r)
+            // to row-major
+            const int T = lay & 0xff, R = lay >> 8;
+            wbuf.resize((size_t)dp32_trace_bytes(m, n));
+            HIP_OK(hipMemcpy(wbuf.data(), r->dtr + r->troff[i], wbuf.size(), hipMemcpyDeviceToHost));
+            for (int row = 0; row < m; ++row) {
+                const int t = row / R, rr = row - t * R;
+                for (int j = 0; j < n; ++j)
+                    traces[i][(size_t)row * n + j] = wbuf[((size_t)(j + t) * T + t) * R + rr];
+            }
         }
     }
     return 0;
@@ -342,6 +436,79 @@ extern "C" int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **trace
         if (rc == 0) rc = resident_fetch(r, out, traces);
         resident_free(r);
     }
+    pthread_mutex_unlock(&g_lock);
+    return rc;
+}
+
+extern "C" int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, const uint8_t *traw,
+                                 phip_out_t *out, phip_cigars_t *cig)
+{
+    if (phip_init() != 0) return -1;
+    if (!b || !out || !cig || !qraw || !traw || b->kind != PHIP_TRACE || b->npairs < 0) {
+        phip_set_error("phip_align_cigars: bad arguments");
+        return -1;
+    }
+    const int np = b->npairs;
+    if (np == 0) return 0;
+    pthread_mutex_lock(&g_lock);
+    int rc = -1;
+    phip_resident_t *r = resident_create(b);
+    char *mem = nullptr;
+    do {
+        if (!r) break;
+        if (resident_run(r, b->alg, PHIP_TRACE, b->sg_flags, b->open, b->gap, b->want_sat_check) != 0)
+            break;
+        // raw residues (upper-cased by the caller) and the op buffers
+        int64_t qtot = 0, ttot = 0, otot = 0;
+        std::vector<int64_t> opoff(np);
+        for (int i = 0; i < np; ++i) {
+            qtot = std::max<int64_t>(qtot, b->qoff[i] + b->qlen[i]);
+            ttot = std::max<int64_t>(ttot, b->toff[i] + b->tlen[i]);
+            opoff[i] = otot;
+            otot += (int64_t)b->qlen[i] + b->tlen[i] + 2;
+        }
+        size_t off = 0;
+        auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
+        const size_t o_q = slot(qtot), o_t = slot(ttot), o_oo = slot((size_t)np * 8),
+                     o_ops = slot((size_t)otot * 4), o_n = slot((size_t)np * 4),
+                     o_b = slot((size_t)np * 8);
+        mem = (char *)dev_alloc(off);
+        if (!mem) break;
+        bool ok = hipMemcpyAsync(mem + o_q, qraw, qtot, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+        ok &= hipMemcpyAsync(mem + o_t, traw, ttot, hipMemcpyHostToDevice, g_stream) == hipSuccess;
+        ok &= hipMemcpyAsync(mem + o_oo, opoff.data(), (size_t)np * 8, hipMemcpyHostToDevice,
+                             g_stream) == hipSuccess;
+        if (!ok) { phip_set_error("phip_align_cigars: upload failed"); break; }
+        CigarArgs a;
+        a.qraw = (const uint8_t *)(mem + o_q);
+        a.traw = (const uint8_t *)(mem + o_t);
+        a.qoff = r->dqoff;
+        a.toff = r->dtoff;
+        a.qlen = r->dqlen;
+        a.tlen = r->dtlen;
+        a.trace = r->dtr;
+        a.troff = r->dtroff;
+        a.tlay = r->dtlay;
+        a.out = r->dout;
+        a.is_sg = b->alg == PHIP_SG;
+        a.npairs = np;
+        a.ops = (uint32_t *)(mem + o_ops);
+        a.opoff = (const int64_t *)(mem + o_oo);
+        a.nops = (int32_t *)(mem + o_n);
+        a.beg = (int32_t *)(mem + o_b);
+        if (cigar_launch(a, g_stream) != 0) break;
+        // the call's device time: DP kernels and the walk
+        if (hipEventRecord(g_ev1, g_stream) != hipSuccess) break;
+        if (resident_fetch(r, out, nullptr) != 0) break;
+        ok = hipMemcpy(cig->nops, a.nops, (size_t)np * 4, hipMemcpyDeviceToHost) == hipSuccess;
+        ok &= hipMemcpy(cig->beg, a.beg, (size_t)np * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok &= hipMemcpy(cig->ops, a.ops, (size_t)otot * 4, hipMemcpyDeviceToHost) == hipSuccess;
+        if (!ok) { phip_set_error("phip_align_cigars: download failed"); break; }
+        for (int i = 0; i < np; ++i) cig->off[i] = opoff[i];
+        rc = 0;
+    } while (0);
+    if (mem) dev_free(mem);
+    if (r) resident_free(r);
     pthread_mutex_unlock(&g_lock);
     return rc;
 }

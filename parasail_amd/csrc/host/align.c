@@ -126,7 +126,8 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
     b.sg_flags = sg_flags;
     b.open = open;
     b.gap = gap;
-    b.want_sat_check = alg != PHIP_SW && width != 32;
+    /* the narrowest width whose saturation the result must report */
+    b.want_sat_check = alg != PHIP_SW && width != 32 ? (width == 16 ? 16 : 8) : 0;
     b.matrix.values = matrix->matrix;
     b.matrix.size = matrix->size;
     b.matrix.length = pssm ? matrix->length : matrix->size;

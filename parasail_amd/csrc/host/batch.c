@@ -7,6 +7,7 @@
  * residues, ships the batch to HBM once and aligns every pair there.
  * Results are the ones the per-pair hip functions return, in flat arrays.
  */
+#include <ctype.h>
 #include <errno.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -68,6 +69,10 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
         fprintf(stderr, "%s: stats functions with pssm require a consensus sequence\n", __func__);
         return -1;
     }
+    if (kind == PHIP_TRACE && out->cigars && !s1) {
+        fprintf(stderr, "%s: CIGAR strings require the query sequence\n", __func__);
+        return -1;
+    }
     if (phip_init() != 0) {
         errno = ENOSYS;
         return -1;
@@ -96,8 +101,33 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
     {
         uint8_t *q = s1 ? (uint8_t *)malloc((size_t)qtot + 1) : NULL;
         uint8_t *t = (uint8_t *)malloc((size_t)ttot + 1);
-        int8_t **tr = NULL;
+        /* CIGARs: traceback and walk on the device, from the residues as
+         * given, upper-cased (cigar_template.c compares them) */
+        uint8_t *qraw = NULL, *traw = NULL;
+        phip_cigars_t cg;
+        memset(&cg, 0, sizeof(cg));
+        const int dev_cigar = kind == PHIP_TRACE && out->cigars;
         if ((s1 && !q) || !t) { free(q); free(t); goto oom; }
+        if (dev_cigar) {
+            int64_t otot = 0;
+            for (int k = 0; k < npairs; ++k) otot += (int64_t)ql[k] + tl[k] + 2;
+            qraw = (uint8_t *)malloc((size_t)qtot + 1);
+            traw = (uint8_t *)malloc((size_t)ttot + 1);
+            cg.ops = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(otot + 1));
+            cg.off = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
+            cg.nops = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
+            cg.beg = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)npairs);
+            if (!qraw || !traw || !cg.ops || !cg.off || !cg.nops || !cg.beg) {
+                free(qraw); free(traw); free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
+                free(q); free(t);
+                goto oom;
+            }
+            for (int k = 0; k < npairs; ++k) {
+                const char *a = s1 + s1_off[k], *bb = s2 + s2_off[k];
+                for (int i = 0; i < ql[k]; ++i) qraw[qo[k] + i] = (uint8_t)toupper((unsigned char)a[i]);
+                for (int j = 0; j < tl[k]; ++j) traw[to[k] + j] = (uint8_t)toupper((unsigned char)bb[j]);
+            }
+        }
         for (int k = 0; k < npairs; ++k) {
             if (s1) {
                 const char *src = s1 + s1_off[k];
@@ -106,18 +136,6 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
             const char *src = s2 + s2_off[k];
             for (int j = 0; j < tl[k]; ++j) t[to[k] + j] = (uint8_t)matrix->mapper[(unsigned char)src[j]];
         }
-        if (kind == PHIP_TRACE && out->cigars) {
-            tr = (int8_t **)calloc((size_t)npairs, sizeof(int8_t *));
-            if (!tr) { free(q); free(t); goto oom; }
-            for (int k = 0; k < npairs; ++k) {
-                tr[k] = (int8_t *)malloc((size_t)ql[k] * tl[k]);
-                if (!tr[k]) {
-                    for (int z = 0; z < k; ++z) free(tr[z]);
-                    free(tr); free(q); free(t);
-                    goto oom;
-                }
-            }
-        }
         phip_batch_t b;
         memset(&b, 0, sizeof(b));
         b.alg = alg;
@@ -125,7 +143,7 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
         b.sg_flags = flags;
         b.open = open;
         b.gap = gap;
-        b.want_sat_check = alg != PHIP_SW && (width == 8 || width == 16);
+        b.want_sat_check = alg != PHIP_SW && (width == 8 || width == 16) ? width : 0;
         b.matrix.values = matrix->matrix;
         b.matrix.size = matrix->size;
         b.matrix.length = pssm ? matrix->length : matrix->size;
@@ -139,13 +157,15 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
         b.t = t;
         b.toff = to;
         b.tlen = tl;
-        int rc = phip_align(&b, res, tr);
+        int rc = dev_cigar ? phip_align_cigars(&b, qraw, traw, res, &cg) : phip_align(&b, res, NULL);
         free(q);
         free(t);
+        free(qraw);
+        free(traw);
         if (rc != 0) {
             snprintf(g_err, sizeof(g_err), "%s", phip_last_error());
             fprintf(stderr, "%s: %s\n", __func__, g_err);
-            if (tr) { for (int k = 0; k < npairs; ++k) free(tr[k]); free(tr); }
+            free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
             free(ql); free(tl); free(qo); free(to); free(res);
             return -1;
         }
@@ -177,14 +197,27 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
             if (out->similar) out->similar[k] = sat ? 0 : o->similar;
             if (out->length) out->length[k] = sat ? 0 : o->length;
             if (out->flag) out->flag[k] = f;
-            if (tr) {
-                out->cigars[k] = sat ? NULL
-                    : phost_cigar_rowmajor(s1 + s1_off[k], ql[k], s2 + s2_off[k], tl[k],
-                                           alg == PHIP_SG, o->end_query, o->end_ref, tr[k]);
-                free(tr[k]);
+            if (dev_cigar) {
+                parasail_cigar_t *c = NULL;
+                if (!sat && cg.nops[k] >= 0) {
+                    c = (parasail_cigar_t *)malloc(sizeof(parasail_cigar_t));
+                    uint32_t *seq = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(cg.nops[k] + 1));
+                    if (!c || !seq) {
+                        free(c);
+                        free(seq);
+                        c = NULL;
+                    } else {
+                        memcpy(seq, cg.ops + cg.off[k], sizeof(uint32_t) * (size_t)cg.nops[k]);
+                        c->seq = seq;
+                        c->len = cg.nops[k];
+                        c->beg_query = cg.beg[2 * k];
+                        c->beg_ref = cg.beg[2 * k + 1];
+                    }
+                }
+                out->cigars[k] = c;
             }
         }
-        free(tr);
+        free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
     }
     free(ql); free(tl); free(qo); free(to); free(res);
     return 0;

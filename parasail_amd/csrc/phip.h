@@ -43,7 +43,8 @@ typedef struct phip_matrix {
 typedef struct phip_batch {
     int alg, kind, sg_flags;
     int open, gap;
-    int want_sat_check;   /* compute PHIP_ST_SAT* bits (per-call widths) */
+    int want_sat_check;   /* 0, or the width (8, 16) whose saturation the
+                           * caller will test: exact extrema are returned */
     phip_matrix_t matrix;
     int npairs;
     /* mapped residue codes (matrix->mapper), host memory.  For a PSSM the
@@ -73,6 +74,21 @@ const char *phip_last_error(void);
  * traces[p] (if non-NULL) receives the row-major int8 trace codes of pair p
  * (qlen*tlen bytes, the LOC_NOVEC layout of cigar.c:156).  0 on success. */
 int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **traces);
+
+/* Traceback of every pair and its CIGAR, both on the device (kind must be
+ * PHIP_TRACE).  qraw/traw: the residues as given, upper-cased, at the
+ * batch's offsets (the walk compares residues, cigar_template.c:104-118).
+ * The caller sizes ops for sum(qlen + tlen + 2) entries, off/nops for
+ * npairs and beg for 2*npairs; pair p's nops[p] ops (-1: no CIGAR) start at
+ * ops + off[p], its start (beg_query, beg_ref) at beg[2p]. */
+typedef struct phip_cigars {
+    uint32_t *ops;
+    int64_t *off;
+    int32_t *nops;
+    int32_t *beg;
+} phip_cigars_t;
+int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, const uint8_t *traw,
+                      phip_out_t *out, phip_cigars_t *cig);
 
 /* Device-resident batch: residues, lengths and results stay in HBM across
  * calls, and the wavefront plan for each scoring is built once.  Score-only

@@ -288,6 +288,40 @@ def test_batch_stats_trace_dna(mats):
             assert tr["cigar"][k] == expc, (name, k)
 
 
+@pytest.mark.parametrize("name", ["sg", "sg_qx", "sg_de", "sg_qb_db", "sw", "nw"])
+def test_batch_stats_trace_wavefront(name, mats):
+    """_stats and _trace through the int32 wavefront kernel (no saturation
+    check), queries over every lane shape and past its range (the general
+    tier in the same batch), mixed-case residues, CIGARs from the device
+    walk."""
+    prng = random.Random(21)
+    pm, om = mats["dna"]
+    qs, ts = [], []
+    for k in range(90):
+        m = prng.choice([1, 3, 17, 64, 250, 257, 320, 321, 640, 641, 1000, 1280, 1281])
+        q = rseq(prng, m, "ACGT")
+        t = mutate(prng, q, "ACGT", sub=0.05, indel=0.02) if k % 3 else \
+            rseq(prng, prng.randint(1, 700), "ACGT")
+        if k % 5 == 0:
+            q = q.lower()
+        qs.append(q)
+        ts.append(t)
+    alg, flags = oracle_alg(name)
+    st = P.align_batch(name, 1, 0, qs, ts, 5, 2, pm)
+    exp, _ = O.batch(alg, O.STATS, [om.map(q) for q in qs], [om.map(t) for t in ts],
+                     5, 2, om, flags)
+    for col, key in enumerate(("score", "end_query", "end_ref", "matches", "similar", "length")):
+        bad = np.nonzero(st[key] != exp[:, col])[0]
+        assert bad.size == 0, (name, key, [(int(k), len(qs[k]), len(ts[k])) for k in bad[:4]])
+    tr = P.align_batch(name, 2, 0, qs, ts, 5, 2, pm, want_cigar=True)
+    for k in range(len(qs)):
+        e, t = O.align(alg, O.TRACE, qs[k], ts[k], 5, 2, om, flags)
+        assert (tr["score"][k], tr["end_query"][k], tr["end_ref"][k]) == e[:3], (name, k)
+        c = O.cigar(qs[k], ts[k], alg == O.SG, e[1], e[2], t)
+        expc = None if c is None else (O.cigar_string(c[0]), c[1], c[2])
+        assert tr["cigar"][k] == expc, (name, k, len(qs[k]), len(ts[k]))
+
+
 def test_banded(mats):
     L = P.lib()
     pm, om = mats["dna"]
