@@ -97,7 +97,7 @@ def per_call_gcups(reps=1):
     return b.cells / best / 1e9, b.cells, best
 
 
-def cpu_baseline(batch, target_s=10.0):
+def cpu_baseline(batch, target_s=15.0):
     """The CPU oracle (serial recurrence per pair, OpenMP over pairs:
     tests/test_openmp.c:383-441) on a bounded sample of the batch."""
     import oracle as O
@@ -191,7 +191,7 @@ def serving(nreq, qps):
                                    "gcups": round(300 * nres / alone_s / 1e9, 1)}}
 
 
-def stats_trace(npairs=100000):
+def stats_trace(peak32, npairs=100000):
     """DNA reads against their windows: 100,000 pairs of a 250 bp read (4%
     substitutions, 0.5% insertions and 0.5% deletions of geometric length,
     mean 2) and its 400 bp window, match 2 mismatch -3, open 5 extend 2;
@@ -217,16 +217,32 @@ def stats_trace(npairs=100000):
     log("stats/trace: trace+cigar %.1f ms kernel, %.1f ms call" % (kms_t, wall_t * 1e3))
     P.lib().parasail_matrix_free(dm)
     ncig = sum(1 for c in rt["cigar"] if c)
+    # int32 VALU instructions per cell of the wavefront kernel: 8 for the
+    # scores, 4 compares, and for _stats 4 selects + 3 adds of the packed
+    # statistics word, for _trace 5 to build the direction code + 1 to pack
+    ops_s, ops_t = 19, 18
+    trace_bytes = sum(WL.dp32_trace_bytes(int(b.qlen[i]), int(b.tlen[i])) for i in range(len(b)))
     return {"workload": "100,000 DNA pairs: 250 bp reads (4% sub, 0.5% ins, 0.5% del, geom "
                         "mean 2) vs 400 bp windows, +2/-3, open 5 extend 2, sg all ends free",
             "pairs": len(b), "cells": b.cells,
             "sg_stats": {"gcups_kernel": round(b.cells / (kms_s / 1e3) / 1e9, 1),
+                         "roofline": {"bound": "valu", "unit": "TOP/s", "ops_per_cell": ops_s,
+                                      "achieved": round(b.cells * ops_s / (kms_s / 1e3) / 1e12, 2),
+                                      "peak": round(peak32 / 1e12, 2),
+                                      "frac": round(b.cells * ops_s / (kms_s / 1e3) / peak32, 4)},
                          "gcups_call": round(b.cells / wall_s / 1e9, 1),
                          "kernel_ms": round(kms_s, 1), "call_ms": round(wall_s * 1e3, 1),
                          "checksum": int(rs["score"].astype(np.int64).sum()
                                          + rs["matches"].astype(np.int64).sum()
                                          + rs["length"].astype(np.int64).sum())},
             "sg_trace_cigar": {"gcups_kernel": round(b.cells / (kms_t / 1e3) / 1e9, 1),
+                               "roofline": {"bound": "valu", "unit": "TOP/s", "ops_per_cell": ops_t,
+                                            "achieved": round(b.cells * ops_t / (kms_t / 1e3) / 1e12, 2),
+                                            "peak": round(peak32 / 1e12, 2),
+                                            "frac": round(b.cells * ops_t / (kms_t / 1e3) / peak32, 4),
+                                            "kernel_ms_includes": "trace DP and the CIGAR walk"},
+                               "trace_bytes_hbm": trace_bytes,
+                               "trace_write_gbs": round(trace_bytes / (kms_t / 1e3) / 1e9, 1),
                                "gcups_call": round(b.cells / wall_t / 1e9, 1),
                                "kernel_ms": round(kms_t, 1), "call_ms": round(wall_t * 1e3, 1),
                                "cigars": ncig}}
@@ -407,7 +423,7 @@ def main():
         if "sw" in per_alg:
             line["batched_over_per_pair_sw"] = round(per_alg["sw"] / g, 1)
     if not a.no_stats:
-        line["stats_trace"] = stats_trace()
+        line["stats_trace"] = stats_trace(cus.value * clk.value * 1e3 * 64)
     if not a.no_long:
         line["long_pairs"] = long_pairs(cus.value * clk.value * 1e3 * 64)
     if not a.no_serve:

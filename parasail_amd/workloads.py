@@ -208,3 +208,14 @@ def banded_cells(m, n, k):
     lo = np.maximum(0, off0 + j)
     hi = np.minimum(m, off0 + j + col_len)
     return int(np.maximum(0, hi - lo).sum())
+
+
+def dp32_trace_bytes(m, n):
+    """HBM bytes of one pair's trace in the wavefront kernel's layout: T
+    lanes (16, 32 or 64, the fewest whose band of <= 20 rows per lane covers
+    the query), R rows per lane (a multiple of 4), (n + T + 1) steps."""
+    for T in (16, 32, 64):
+        if m <= T * 20:
+            R = max(4, ((m + T - 1) // T + 3) & ~3)
+            return (n + T + 1) * T * R
+    return m * n
