@@ -24,6 +24,13 @@ __device__ __forceinline__ int max(int a, int b)
     return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(s2v, a),
                                                             __builtin_bit_cast(s2v, b)));
 }
+// unsigned saturating subtraction of both halves (clamps at 0)
+__device__ __forceinline__ int subu(int a, int b)
+{
+    typedef unsigned short u2v __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(int, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2v, a),
+                                                                __builtin_bit_cast(u2v, b)));
+}
 __device__ __forceinline__ int lo(int v) { return (int)(short)(v & 0xffff); }
 __device__ __forceinline__ int hi(int v) { return v >> 16; }
 __host__ __device__ __forceinline__ int pack(int l, int h) { return (l & 0xffff) | (h << 16); }

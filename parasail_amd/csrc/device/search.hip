@@ -10,10 +10,11 @@
 // Kernel layout (k_search): as the pair wavefront kernel (wave.hip) -- two
 // targets per job in the two int16 halves of every VGPR, a chain of T lanes
 // per job, R query rows per lane -- but all jobs share the query, so the
-// substitution scores come from a query profile (code-major, W+open per
-// query row) in LDS or L2: a lane reads its R rows for the column's residue
-// with R/4 8-byte loads per half and interleaves the halves with one
-// v_perm_b32 per row.
+// substitution scores come from a query profile (code-major, W per query
+// row) in LDS or L2: a lane reads its R rows for the column's residue with
+// R/4 8-byte loads per half and interleaves the halves with one v_perm_b32
+// per row.  Values are kept clamped at 0 (H, max(H - open, 0), E, F), so
+// the local clamp costs no instruction of its own.
 #include <pthread.h>
 #include <string.h>
 #include <time.h>
@@ -55,12 +56,11 @@ struct SearchCfg {
     int w0, nw;               // this launch's queue entries [w0, w0 + nw)
     int *next_wave;
     const uint16_t *colcode;
-    const short *prof;        // (size+1) code rows of mp int16 (W+open; W_NEG pads)
+    const short *prof;        // (size+1) code rows of mp int16 (W; W_NEG pads)
     int mp, profn;            // rows per code; profile size in dwords
     int nb16, nb64;           // bands for T = 16 and T = 64 chains
     int2 *bandbuf;
-    const int2 *toprow;       // (H-open, F) of DP row 0 for SW: (-open, -inf)
-    int2 *dump;
+    const int2 *toprow;       // (H, F) of DP row 0 for SW, clamped: (0, 0)
     phip_out_t *out;          // per target
     int32_t *redo;
     int *nredo;
@@ -74,7 +74,7 @@ namespace {
 // All bands of one queue entry, R rows per lane (R a multiple of 4).
 template <int R, bool LDSP>
 __device__ __forceinline__ void search_wave(const SearchCfg &c, const DbWave &wd, const short *prof,
-                                            int lane, int2 *dump, short *snp)
+                                            int lane, short *snp)
 {
     const int T = wd.T;
     const int t = lane & (T - 1);
@@ -91,14 +91,18 @@ __device__ __forceinline__ void search_wave(const SearchCfg &c, const DbWave &wd
     int cS[2] = {-1, -1}, cC[2] = {0, 0}, cR[2] = {0, 0};
     for (int b = 0; b < nb; ++b) {
         const int r0 = b * T * R + t * R;
-        int HL[R], E[R];
+        // every value is clamped at 0 (values below 0 never reach a local
+        // score): Hs = H, HL = max(H - open, 0), E and F >= 0, so
+        // H = max(diag + W, E, F) needs no clamp of its own
+        int Hs[R], HL[R], E[R];
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-            HL[i] = pk::pack(-open, -open);  // column 0 of SW: H = 0
-            E[i] = PK_NEG;
+            Hs[i] = 0;  // column 0 of SW: H = 0
+            HL[i] = 0;
+            E[i] = 0;
         }
-        int prevRH = pk::pack(-open, -open);
-        int botH = PK_NEG, botF = PK_NEG;
+        int prevRH = 0;
+        int botH = 0, botF = 0;
         int best = (int)0xFFFFFFFF;
         int bcolA = 0, bcolB = 0;
         const uint16_t *ccl = c.colcode + J.ccoff + (T - t);
@@ -127,35 +131,34 @@ __device__ __forceinline__ void search_wave(const SearchCfg &c, const DbWave &wd
                 const int rF = t == 0 ? bx.y : xF;
                 int t1 = pk::add(prevRH, W[0]);
                 prevRH = rH;
-                int up = rH, F = rF;
+                int up = pk::subu(rH, openv), F = rF;
                 int cm = 0;
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
-                    E[i] = pk::max(pk::sub(E[i], gapv), HL[i]);
-                    const int t1n = i + 1 < R ? pk::add(HL[i], W[i + 1]) : 0;
-                    F = pk::max(pk::sub(F, gapv), up);
-                    int h = pk::max(pk::max(t1, E[i]), F);
-                    h = pk::max(h, 0);
-                    HL[i] = pk::sub(h, openv);
+                    E[i] = pk::max(pk::subu(E[i], gapv), HL[i]);
+                    const int t1n = i + 1 < R ? pk::add(Hs[i], W[i + 1]) : 0;
+                    F = pk::max(pk::subu(F, gapv), up);
+                    const int h = pk::max(pk::max(t1, E[i]), F);
+                    Hs[i] = h;
+                    HL[i] = pk::subu(h, openv);
                     up = HL[i];
                     t1 = t1n;
                     cm = i == 0 ? h : pk::max(cm, h);
                 }
-                botH = HL[R - 1];
+                botH = Hs[R - 1];
                 botF = F;
-                int2 *dst = wr_lane ? bbr + j : dump;
-                *dst = make_int2(botH, botF);
+                if (wr_lane) bbr[j] = make_int2(botH, botF);
                 const int nbst = pk::max(best, cm);
                 if (nbst != best) {
                     const int d = nbst ^ best;
                     if (d & 0xffff) {
 #pragma unroll
-                        for (int i = 0; i < R; ++i) snp[i * kThreads] = (short)HL[i];
+                        for (int i = 0; i < R; ++i) snp[i * kThreads] = (short)Hs[i];
                         bcolA = j;
                     }
                     if (d >> 16) {
 #pragma unroll
-                        for (int i = 0; i < R; ++i) snp[(kSnapRows + i) * kThreads] = (short)(HL[i] >> 16);
+                        for (int i = 0; i < R; ++i) snp[(kSnapRows + i) * kThreads] = (short)(Hs[i] >> 16);
                         bcolB = j;
                     }
                     best = nbst;
@@ -178,7 +181,7 @@ __device__ __forceinline__ void search_wave(const SearchCfg &c, const DbWave &wd
                 int row = -1;
 #pragma unroll
                 for (int i = R - 1; i >= 0; --i) {
-                    const int v = (int)snp[(h * kSnapRows + i) * kThreads] + open;
+                    const int v = (int)snp[(h * kSnapRows + i) * kThreads];
                     if (r0 + i < c.m && v == S) row = r0 + i;
                 }
                 if (row >= 0 && (S > cS[h] || (S == cS[h] && col < cC[h]))) {
@@ -228,14 +231,13 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2
     }
     short *const snp = (short *)(lds + off) + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    int2 *const dump = c.dump + (threadIdx.x + blockIdx.x * blockDim.x);
     for (;;) {
         int wv = 0;
         if (lane == 0) wv = atomicAdd(c.next_wave, 1);
         wv = __builtin_amdgcn_readfirstlane(wv);
         if (wv >= c.nw) break;
         const DbWave wd = c.waves[c.w0 + wv];
-        search_wave<R, LDSP>(c, wd, prof, lane, dump, snp);
+        search_wave<R, LDSP>(c, wd, prof, lane, snp);
     }
 }
 
@@ -485,7 +487,7 @@ struct Req {
     phip_out_t *dout = nullptr;
     int32_t *dredo = nullptr;
     int *dcnt = nullptr;         // [0] nredo, [1..] chunk queue counters
-    int2 *dbb = nullptr, *ddump = nullptr, *dtop = nullptr;
+    int2 *dbb = nullptr, *dtop = nullptr;
     uint8_t *dq = nullptr;
     int *dmat = nullptr;
     int *dscratch = nullptr;
@@ -577,7 +579,7 @@ int req_prepare(Req *r, const phip_db_t *db, const phip_query_t *q, int k)
                                         : q->mx.values[(size_t)q->codes[i] * size + c];
             vmax = std::max(vmax, v);
             if (v + q->open > 16000 || v < -16000) fits = false;
-            r->hprof[(size_t)c * r->mp + i] = (short)std::max(-16000, std::min(16000, v + q->open));
+            r->hprof[(size_t)c * r->mp + i] = (short)std::max(-16000, std::min(16000, v));
         }
     // int16 tier: targets whose running maximum passes this limit are re-run
     // exactly in int32 (all of them when the scores do not fit int16)
@@ -628,7 +630,7 @@ int req_alloc(Req *r, const phip_db_t *db)
     const size_t o_prof = slot(r->hprof.size() * 2), o_out = slot((size_t)n * sizeof(phip_out_t)),
                  o_redo = slot((size_t)n * 4), o_cnt = slot((size_t)(nch + 2) * 4),
                  o_bb = slot(r->needbb ? (size_t)db->bbtot * 8 : 0),
-                 o_dump = slot((size_t)search_blocks() * kThreads * 8), o_top = slot((size_t)db->ntop * 8),
+                 o_top = slot((size_t)db->ntop * 8),
                  o_q = slot(r->qcodes.size()), o_mat = slot(r->pssm.size() * 4),
                  o_scr = slot(per_lane * 64), o_keys = slot((size_t)n * 8),
                  o_sorted = slot((size_t)n * 8), o_temp = slot(temp), o_hits = slot((size_t)r->k * sizeof(phip_hit_t));
@@ -641,7 +643,6 @@ int req_alloc(Req *r, const phip_db_t *db)
     r->dredo = (int32_t *)(b + o_redo);
     r->dcnt = (int *)(b + o_cnt);
     r->dbb = (int2 *)(b + o_bb);
-    r->ddump = (int2 *)(b + o_dump);
     r->dtop = (int2 *)(b + o_top);
     r->dq = r->qcodes.empty() ? nullptr : (uint8_t *)(b + o_q);
     r->dmat = (int *)(b + o_mat);
@@ -678,7 +679,7 @@ int req_start(Req *r, const phip_db_t *db, hipStream_t st)
     if (r->dq) HIP_OK(hipMemcpyAsync(r->dq, r->qcodes.data(), r->qcodes.size(), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(r->dmat, r->pssm.data(), r->pssm.size() * 4, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, r->dcnt, (int64_t)(nch + 2), 0);
-    const int2 top = make_int2(pk::pack(-r->open, -r->open), PK_NEG);
+    const int2 top = make_int2(0, 0);  // H = 0, F clamped at 0
     std::vector<int2> htop((size_t)db->ntop, top);
     HIP_OK(hipMemcpyAsync(r->dtop, htop.data(), htop.size() * 8, hipMemcpyHostToDevice, st));
     // the pageable sources above must outlive the copies
@@ -703,7 +704,6 @@ int req_launch_chunk(Req *r, const phip_db_t *db, int ch, int blocks, hipStream_
     c.nb64 = r->nb64;
     c.bandbuf = r->dbb;
     c.toprow = r->dtop;
-    c.dump = r->ddump;
     c.out = r->dout;
     c.redo = r->dredo;
     c.nredo = r->dcnt;

@@ -15,10 +15,14 @@
 //   * queries longer than one band run as several bands; the bottom row of
 //     a band goes through a per-job HBM row written by lane T-1 and read by
 //     lane 0 of the next band (band 0 reads the DP's row 0 from a table);
-//   * H is kept as H-open ("Hopen"): the diagonal term H_diag + W becomes
-//     Hopen_diag + (W + open), so E, F and the next diagonal share one
-//     subtraction.  Scores come from an LDS copy of the substitution table
-//     already offset by +open.
+//   * NW/SG keep H as H-open ("Hopen"): the diagonal term H_diag + W
+//     becomes Hopen_diag + (W + open), so E, F and the next diagonal share
+//     one subtraction (the LDS table holds W + open).  SW keeps every value
+//     clamped at 0 instead -- H, max(H - open, 0) and E, F >= 0 with
+//     unsigned saturating subtraction -- so max(diag + W, E, F) is already
+//     the local max(..., 0) and needs no clamp of its own (the table holds
+//     W); values below 0 never reach a local score, so the result is the
+//     same.
 //   * one persistent launch per pass: waves take work descriptors off a
 //     queue ordered by decreasing latency, so the longest jobs start first
 //     and the short ones fill the machine around them; R (rows per lane) is
@@ -55,6 +59,13 @@ __device__ __forceinline__ int pk_max(int a, int b)
 {
     return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(s2v, a),
                                                             __builtin_bit_cast(s2v, b)));
+}
+// unsigned saturating subtraction of both halves (clamps at 0)
+__device__ __forceinline__ int pk_subu(int a, int b)
+{
+    typedef unsigned short u2v __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(int, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2v, a),
+                                                                __builtin_bit_cast(u2v, b)));
 }
 __device__ __forceinline__ int lo16(int v) { return (int)(short)(v & 0xffff); }
 __device__ __forceinline__ int hi16(int v) { return v >> 16; }
@@ -102,7 +113,6 @@ struct WaveCfg {
     int VIRT, PADQ;
     int *next_wave;           // work-queue counter
     const uint8_t *q;
-    int2 *dump;               // per-thread slot for stores a lane does not need
     int2 *bandbuf;
     const int2 *toprow;       // DP row 0 as (H-open, F) per column, both halves
     int4 *cand;
@@ -121,7 +131,7 @@ namespace {
 // All bands of one queue entry, R rows per lane.
 template <int R, bool SW>
 __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, const char *tabb,
-                                         int lane, int2 *dump, short *snp)
+                                         int lane, short *snp)
 {
     const int T = wd.T;
     const int t = lane & (T - 1);
@@ -143,7 +153,11 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
     for (int b = 0; b < wd.nb; ++b) {
         const bool band_live = live && b < J.nb;
         int qo[R];  // table row offsets: half A in bits 0-15, half B in 16-31
+        // NW/SG: HL = H - open.  SW: every value is clamped at 0 (values
+        // below 0 never reach a local score), Hs = H, HL = max(H - open, 0),
+        // E and F >= 0, so H = max(diag + W, E, F) needs no clamp of its own
         int HL[R], E[R];
+        int Hs[SW ? R : 1];
         int first_row[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) first_row[h] = b * T * R + t * R - J.s[h];
@@ -169,12 +183,20 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         };
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-            HL[i] = pack16(left(first_row[0] + i), left(first_row[1] + i));
-            E[i] = PK_NEG;
+            if (SW) {
+                HL[i] = 0;
+                Hs[i] = 0;
+                E[i] = 0;
+            } else {
+                HL[i] = pack16(left(first_row[0] + i), left(first_row[1] + i));
+                E[i] = PK_NEG;
+            }
         }
-        int prevRH = pack16(left(first_row[0] - 1), left(first_row[1] - 1));
-        if (b == 0 && t == 0) prevRH = pack16(-open, -open);  // H[0][0] = 0
-        int botH = PK_NEG, botF = PK_NEG;
+        // diagonal of the lane's first row: SW the H of the row above (0 in
+        // column 0), NW/SG its H-open
+        int prevRH = SW ? 0 : pack16(left(first_row[0] - 1), left(first_row[1] - 1));
+        if (!SW && b == 0 && t == 0) prevRH = pack16(-open, -open);  // H[0][0] = 0
+        int botH = SW ? 0 : PK_NEG, botF = SW ? 0 : PK_NEG;
         int best = (int)0xFFFFFFFF;
         // NW/SG: the lane's rows at each half's last column (SW keeps its
         // snapshot, the rows at the column of the running maximum, in LDS)
@@ -214,26 +236,36 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 const int rF = t == 0 ? bx.y : xF;
                 int t1 = pk_add(prevRH, W[0]);
                 prevRH = rH;
-                int up = rH, F = rF;
+                // SW passes H between lanes and bands, the others H - open
+                int up = SW ? pk_subu(rH, openv) : rH, F = rF;
                 int cm = 0;
-                // each HL[i] is read (E, next row's diagonal) before it is
-                // rewritten, so it stays in its register across steps
+                // each HL[i] (SW: Hs[i]) is read (E, next row's diagonal)
+                // before it is rewritten, so it stays in its register
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
-                    E[i] = pk_max(pk_sub(E[i], gapv), HL[i]);
-                    const int t1n = i + 1 < R ? pk_add(HL[i], W[i + 1]) : 0;
-                    F = pk_max(pk_sub(F, gapv), up);
-                    int h = pk_max(pk_max(t1, E[i]), F);
-                    if (SW) h = pk_max(h, 0);
-                    HL[i] = pk_sub(h, openv);
-                    up = HL[i];
-                    t1 = t1n;
-                    if (SW) cm = i == 0 ? h : pk_max(cm, h);
+                    if (SW) {
+                        E[i] = pk_max(pk_subu(E[i], gapv), HL[i]);
+                        const int t1n = i + 1 < R ? pk_add(Hs[i], W[i + 1]) : 0;
+                        F = pk_max(pk_subu(F, gapv), up);
+                        const int h = pk_max(pk_max(t1, E[i]), F);
+                        Hs[i] = h;
+                        HL[i] = pk_subu(h, openv);
+                        up = HL[i];
+                        t1 = t1n;
+                        cm = i == 0 ? h : pk_max(cm, h);
+                    } else {
+                        E[i] = pk_max(pk_sub(E[i], gapv), HL[i]);
+                        const int t1n = i + 1 < R ? pk_add(HL[i], W[i + 1]) : 0;
+                        F = pk_max(pk_sub(F, gapv), up);
+                        const int h = pk_max(pk_max(t1, E[i]), F);
+                        HL[i] = pk_sub(h, openv);
+                        up = HL[i];
+                        t1 = t1n;
+                    }
                 }
-                botH = HL[R - 1];
+                botH = SW ? Hs[R - 1] : HL[R - 1];
                 botF = F;
-                int2 *dst = wr_lane ? bbr + j : dump;
-                *dst = make_int2(botH, botF);
+                if (wr_lane) bbr[j] = make_int2(botH, botF);
                 if (SW) {
                     const int nbst = pk_max(best, cm);
                     if (nbst != best) {
@@ -242,12 +274,13 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                         const int d = nbst ^ best;
                         if (d & 0xffff) {
 #pragma unroll
-                            for (int i = 0; i < R; ++i) snp[i * 256] = (short)HL[i];
+                            for (int i = 0; i < R; ++i) snp[i * 256] = (short)Hs[SW ? i : 0];
                             bcolA = j;
                         }
                         if (d >> 16) {
 #pragma unroll
-                            for (int i = 0; i < R; ++i) snp[(kSnapRows + i) * 256] = (short)(HL[i] >> 16);
+                            for (int i = 0; i < R; ++i)
+                                snp[(kSnapRows + i) * 256] = (short)(Hs[SW ? i : 0] >> 16);
                             bcolB = j;
                         }
                         best = nbst;
@@ -280,7 +313,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
 #pragma unroll
                 for (int i = R - 1; i >= 0; --i) {
                     const int g = first_row[h] + i;
-                    const int v = (int)snp[(h * kSnapRows + i) * 256] + open;
+                    const int v = (int)snp[(h * kSnapRows + i) * 256];
                     if (g >= 0 && g < J.m[h] && v == S) row = g;
                 }
                 if (row >= 0 && (S > cS[h] || (S == cS[h] && col < cC[h]))) {
@@ -332,7 +365,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const int lane = threadIdx.x & 63;
     const char *tabb = (const char *)tab32 + (lane % c.ncopy) * 4;
     short *const snp = (short *)(tab32 + ((c.tabn + 3) & ~3)) + threadIdx.x;
-    int2 *const dump = c.dump + (threadIdx.x + blockIdx.x * blockDim.x);
     // persistent: each wave takes the next entry off the queue (entries are
     // ordered by decreasing latency, so the tail is short)
     for (;;) {
@@ -343,16 +375,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         const WaveDesc wd = c.waves[wv];
         if (wd.prio) __builtin_amdgcn_s_setprio(2);
         switch (wd.R) {
-        case 4: wave_run<4, SW>(c, wd, tabb, lane, dump, snp); break;
-        case 6: wave_run<6, SW>(c, wd, tabb, lane, dump, snp); break;
-        case 8: wave_run<8, SW>(c, wd, tabb, lane, dump, snp); break;
-        case 10: wave_run<10, SW>(c, wd, tabb, lane, dump, snp); break;
-        case 12: wave_run<12, SW>(c, wd, tabb, lane, dump, snp); break;
-        case 14: wave_run<14, SW>(c, wd, tabb, lane, dump, snp); break;
-        case 16: wave_run<16, SW>(c, wd, tabb, lane, dump, snp); break;
-        case 18: wave_run<18, SW>(c, wd, tabb, lane, dump, snp); break;
-        default: wave_run<20, SW>(c, wd, tabb, lane, dump, snp); break;
-        }
+        case 4: wave_run<4, SW>(c, wd, tabb, lane, snp); break;
+        case 6: wave_run<6, SW>(c, wd, tabb, lane, snp); break;
+        case 8: wave_run<8, SW>(c, wd, tabb, lane, snp); break;
+        case 10: wave_run<10, SW>(c, wd, tabb, lane, snp); break;
+        case 12: wave_run<12, SW>(c, wd, tabb, lane, snp); break;
+        case 14: wave_run<14, SW>(c, wd, tabb, lane, snp); break;
+        case 16: wave_run<16, SW>(c, wd, tabb, lane, snp); break;
+        case 18: wave_run<18, SW>(c, wd, tabb, lane, snp); break;
+        default: wave_run<20, SW>(c, wd, tabb, lane, snp); break;
+    }
         if (wd.prio) __builtin_amdgcn_s_setprio(0);
     }
 }
@@ -551,7 +583,7 @@ struct WavePlan {
     WaveDesc *dwaves = nullptr;
     short *dtab = nullptr;
     uint32_t *dcc = nullptr;
-    int2 *dbb = nullptr, *dtop = nullptr, *ddump = nullptr;
+    int2 *dbb = nullptr, *dtop = nullptr;
     int4 *dcd = nullptr;
     PairRef *drefs = nullptr;
     int32_t *dslow = nullptr, *dredo = nullptr;
@@ -643,7 +675,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
                 int v = W_NEG;
                 if (col >= P->PADT || r == P->PADQ) v = W_NEG;
                 else if (r == P->VIRT) v = zero_top ? dp.open : W_NEG;
-                else v = mx.values[(size_t)r * size + col] + dp.open;
+                else v = mx.values[(size_t)r * size + col] + (sw ? 0 : dp.open);
                 for (int cp = 0; cp < P->ncopy; ++cp) {
                     size_t byte = (size_t)r * P->qstride + (size_t)(col >> 1) * P->ncopy * 4 +
                                   (col & 1) * 2 + cp * 4;
@@ -767,8 +799,9 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     const int ntop = maxw + 64 + 16;
     std::vector<int2> top(ntop);
     for (int j = 0; j < ntop; ++j) {
-        const int v = zero_top ? -dp.open : clamp16(-dp.open - j * dp.gap - dp.open);
-        top[j] = make_int2(pack16(v, v), PK_NEG);
+        // SW: H = 0 and F clamped at 0; NW/SG: H - open and F = -inf
+        const int v = sw ? 0 : zero_top ? -dp.open : clamp16(-dp.open - j * dp.gap - dp.open);
+        top[j] = make_int2(pack16(v, v), sw ? 0 : PK_NEG);
     }
     // int32-tier scratch: rows of up to maxn+1 columns per lane
     const size_t per_lane = (size_t)(P->maxn + 1) * 2 * sizeof(int);
@@ -780,7 +813,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     size_t o_jobs = slot(nj * sizeof(WaveJob)), o_waves = slot(qw.size() * sizeof(WaveDesc)),
            o_tab = slot(htab.size() * 2), o_cc = slot((size_t)ccoff * 4),
            o_clen = slot(nj * 4), o_cT = slot(nj * 4), o_bb = slot((size_t)bboff * 8),
-           o_top = slot((size_t)ntop * 8), o_dump = slot((size_t)1024 * 256 * 8),
+           o_top = slot((size_t)ntop * 8),
            o_cd = slot((size_t)cdoff * 16), o_refs = slot(refs.size() * sizeof(PairRef)),
            o_slow = slot(slow.size() * 4), o_redo = slot(refs.size() * 4), o_count = slot(64),
            o_scr = slot(per_lane * P->simple_lanes), o_long = slot(P->lp.bytes);
@@ -793,7 +826,6 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->dcc = (uint32_t *)(b + o_cc);
     P->dbb = (int2 *)(b + o_bb);
     P->dtop = (int2 *)(b + o_top);
-    P->ddump = (int2 *)(b + o_dump);
     P->dcd = (int4 *)(b + o_cd);
     P->drefs = (PairRef *)(b + o_refs);
     P->dslow = (int32_t *)(b + o_slow);
@@ -868,7 +900,6 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
         cfg.VIRT = P->VIRT;
         cfg.PADQ = P->PADQ;
         cfg.q = dp.q;
-        cfg.dump = P->ddump;
         cfg.bandbuf = P->dbb;
         cfg.toprow = P->dtop;
         cfg.cand = P->dcd;
