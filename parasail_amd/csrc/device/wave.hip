@@ -252,7 +252,6 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                         HL[i] = pk_subu(h, openv);
                         up = HL[i];
                         t1 = t1n;
-                        cm = i == 0 ? h : pk_max(cm, h);
                     } else {
                         E[i] = pk_max(pk_sub(E[i], gapv), HL[i]);
                         const int t1n = i + 1 < R ? pk_add(HL[i], W[i + 1]) : 0;
@@ -267,6 +266,17 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 botF = F;
                 if (wr_lane) bbr[j] = make_int2(botH, botF);
                 if (SW) {
+                    // the rows' maximum as a balanced tree (independent
+                    // maxima issue back to back; a serial chain would wait
+                    // on every result)
+                    int mx[R];
+#pragma unroll
+                    for (int i = 0; i < R; ++i) mx[i] = Hs[SW ? i : 0];
+#pragma unroll
+                    for (int w = 1; w < R; w *= 2)
+#pragma unroll
+                        for (int i = 0; i + w < R; i += 2 * w) mx[i] = pk_max(mx[i], mx[i + w]);
+                    cm = mx[0];
                     const int nbst = pk_max(best, cm);
                     if (nbst != best) {
                         // the half whose maximum rose stores its rows in LDS
