@@ -282,7 +282,7 @@ def long_pairs(peak32):
             "pairs": len(b), "cells": b.cells,
             "nw_int32": {"gcups": round(b.cells / (kms / 1e3) / 1e9, 1),
                          "kernel_ms": round(kms, 1), "call_ms": round(wall * 1e3, 1),
-                         "pairs_int32": kb, "kernel": "k_nwlong<16> (15 bands per workgroup)",
+                         "pairs_int32": kb, "kernel": "k_nwlong<16> (16 bands per workgroup)",
                          "roofline": {"bound": "valu", "achieved": round(ach / 1e12, 2),
                                       "peak": round(peak32 / 1e12, 2), "unit": "TOP/s",
                                       "frac": round(ach / peak32, 4),

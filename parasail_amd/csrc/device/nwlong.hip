@@ -31,7 +31,7 @@
 
 namespace {
 
-constexpr int LW = 15;     // waves (bands in flight) per workgroup
+constexpr int LW = 16;     // waves (bands in flight) per workgroup: 4 per SIMD
 constexpr int LK = 128;    // steps between barriers
 constexpr int LL = 2 * LK; // lag between consecutive bands
 constexpr int LRING = 512; // ring slots per band boundary (power of two)
@@ -42,6 +42,7 @@ constexpr int kGroupRows = LW * 64 * kLongR;
 // bound on the polls of one wait (each sleeps ~0.2 us); legitimate waits
 // are a few thousand steps of the group above, well under a second
 constexpr int kSpinLimit = 1 << 23;
+constexpr int kPubPhases = 4;  // barriers between progress publications (power of two)
 
 __device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, false); }
 
@@ -73,8 +74,11 @@ struct LongCfg {
 };
 
 // LDS: table of W+open ((size+1) rows: codes, then a pad row), then the
-// LW-1 rings of (H-open, F).
-template <int R>
+// LW-1 rings of (H-open, F).  RP: an alphabet of at most 8 codes whose
+// W+open all lie in [0, 15] (DNA): each row's scores sit in one register as
+// 4-bit fields, so a lookup is one bit-field extract instead of an LDS read
+// (rows past the query read 0: their values are never used).
+template <int R, bool RP>
 __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
 {
     extern __shared__ int lds[];
@@ -115,16 +119,19 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
         const int G = n + 63 + LL * (LW - 1);
         const int nph = (G + LK - 1) / LK;
         int score = 0;
-        bool have = false;
 
         // band 0 of a later group: before steps [ph*LK, ph*LK+LK), the rows
-        // it reads there and two steps beyond (prefetch) must be final
+        // it reads there and two steps beyond (prefetch) must be final.  The
+        // count last acquired is kept: an acquire (which invalidates this
+        // CU's caches) is made only when the rows needed pass it
+        int seen = 0;
         auto wait_rows = [&](int ph) {
             if (gin && w == 0 && t == 0) {
                 const int need = min(n, ph * LK + LK + 2);
+                if (need <= seen) return;
                 const int *pin = c.prog + item - 1;
                 int spins = 0;
-                while (__hip_atomic_load(pin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                while ((seen = __hip_atomic_load(pin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need) {
                     if (__hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
                         ++spins > kSpinLimit) {
                         __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -141,6 +148,12 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
         for (int i = 0; i < R; ++i) {
             const int row = first_row + i;
             qo[i] = (row < m ? (int)s1[row] : msize) * msize;
+            if (RP) {
+                int pf = 0;
+                if (row < m)
+                    for (int k = 0; k < msize; ++k) pf |= lds[qo[i] + k] << (4 * k);
+                qo[i] = pf;
+            }
             HL[i] = left(row);
             E[i] = LNEG;
         }
@@ -169,33 +182,35 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
             if (j >= 0 && j < n) {
                 const int code = tq;
                 int W[R];
+                if (RP) {
+                    const int sh = code * 4;
 #pragma unroll
-                for (int i = 0; i < R; ++i) W[i] = lds[qo[i] + code];
+                    for (int i = 0; i < R; ++i) W[i] = (int)__builtin_amdgcn_ubfe((unsigned)qo[i], sh, 4);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < R; ++i) W[i] = lds[qo[i] + code];
+                }
                 const int rH = t == 0 ? bx.x : xH;
                 const int rF = t == 0 ? bx.y : xF;
-                int diag = prevRH;
+                int t1 = prevRH + W[0];
                 prevRH = rH;
                 int up = rH, F = rF;
+                // the next row's diagonal term is taken before HL[i] is
+                // rewritten, so every HL[i] stays in its register
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
-                    const int t1 = diag + W[i];
                     E[i] = max(E[i] - gap, HL[i]);
+                    const int t1n = i + 1 < R ? HL[i] + W[i + 1] : 0;
                     F = max(F - gap, up);
                     const int h = max(max(t1, E[i]), F);
-                    diag = HL[i];
                     HL[i] = h - open;
                     up = HL[i];
+                    t1 = t1n;
                 }
                 botH = HL[R - 1];
                 botF = F;
                 if (t == 63 && w < LW - 1) ring_out[j & (LRING - 1)] = make_int2(botH, botF);
                 if (pub) gout[j] = make_int2(botH, botF);
-                if (b == bm && t == tm && j == n - 1) {
-#pragma unroll
-                    for (int i = 0; i < R; ++i)
-                        if (i == rm) score = HL[i] + open;
-                    have = true;
-                }
             }
             bx = ld_in(s + 2);
             tq = ld_t(j + 2);
@@ -207,8 +222,10 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
                     step(sb + k, in0, tc0);
                     step(sb + k + 1, in1, tc1);
                 }
-                if (pub) {
+                if (pub && ((ph & (kPubPhases - 1)) == kPubPhases - 1 || ph + 1 == nph)) {
                     // columns of the bottom row stored so far, then published
+                    // (every kPubPhases barriers: each release writes back
+                    // this CU's L2)
                     const int done = min(n, max(0, (ph + 1) * LK - LL * (LW - 1) - 63));
                     __hip_atomic_store(c.prog + item, done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 }
@@ -216,7 +233,12 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
             if (ph + 1 < nph) wait_rows(ph + 1);
             __syncthreads();
         }
-        if (have) {
+        // a lane stops updating after its last column, so its rows now hold
+        // column n-1: the score is H[m-1][n-1] (nw.c:155-157)
+        if (b == bm && t == tm) {
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (i == rm) score = HL[i] + open;
             phip_out_t o;
             memset(&o, 0, sizeof(o));
             o.score = score;
@@ -252,15 +274,21 @@ size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 bool nwlong_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int m, int n)
 {
-    // the table and the rings must fit the 64 KiB of LDS a workgroup gets
+    // the table and the LW-1 rings must fit the 64 KiB of LDS a workgroup gets
     return alg == PHIP_NW && kind == PHIP_SCORE && !check && !mx.is_pssm && mx.size <= 24 &&
            m >= 1024 && n >= 64 && (int64_t)m * n >= ((int64_t)1 << 24);
 }
 
-bool nwlong_plan(const HostBatch &hb, const std::vector<int> &pairs, LongPlan &lp)
+bool nwlong_plan(const HostBatch &hb, const std::vector<int> &pairs, int open, LongPlan &lp)
 {
     lp = LongPlan();
     if (pairs.empty()) return true;
+    // scores in registers when every W + open fits a 4-bit field
+    lp.rp = hb.mx->size <= 8;
+    for (int x = 0; lp.rp && x < hb.mx->size * hb.mx->size; ++x) {
+        const int v = hb.mx->values[x] + open;
+        if (v < 0 || v > 15) lp.rp = false;
+    }
     const int fill = LL * (LW - 1) + 64;
     // longest chain first: a pair's last group ends about n + groups * fill
     // steps after its first starts
@@ -293,7 +321,9 @@ bool nwlong_plan(const HostBatch &hb, const std::vector<int> &pairs, LongPlan &l
         cus = 256;
     const int tabn = (hb.mx->size + 1) * hb.mx->size;
     lp.lds = (size_t)((tabn + 1) & ~1) * 4 + (size_t)(LW - 1) * LRING * 8;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nwlong<kLongR>, LW * 64, lp.lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lp.rp ? k_nwlong<kLongR, true>
+                                                                     : k_nwlong<kLongR, false>,
+                                                     LW * 64, lp.lds) !=
             hipSuccess || per_cu <= 0)
         per_cu = 1;
     lp.nitems = (int)items.size();
@@ -332,7 +362,8 @@ int nwlong_launch(const DPParams &dp, const LongPlan &lp, char *region, hipStrea
     c.out = dp.out;
     hipLaunchKernelGGL(k_nwlong_zero, dim3(std::min((lp.nitems + 2 + 255) / 256, 1024)), dim3(256), 0,
                        st, c.prog, lp.nitems + 2);
-    hipLaunchKernelGGL(k_nwlong<kLongR>, dim3(lp.grid), dim3(LW * 64), lp.lds, st, c);
+    if (lp.rp) hipLaunchKernelGGL((k_nwlong<kLongR, true>), dim3(lp.grid), dim3(LW * 64), lp.lds, st, c);
+    else hipLaunchKernelGGL((k_nwlong<kLongR, false>), dim3(lp.grid), dim3(LW * 64), lp.lds, st, c);
     hipLaunchKernelGGL(k_nwlong_check, dim3(std::min((lp.nitems + 255) / 256, 1024)), dim3(256), 0, st,
                        c.ctl, c.items, lp.nitems, c.out);
     HIP_OK(hipGetLastError());

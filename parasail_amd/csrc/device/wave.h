@@ -43,7 +43,8 @@ struct LongPlan {
     size_t bytes = 0, lds = 0;
     size_t o_items = 0, o_prog = 0, o_gbuf = 0;
     int nitems = 0, grid = 0;
+    bool rp = false;  // scores in registers (small alphabet)
 };
 bool nwlong_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int m, int n);
-bool nwlong_plan(const HostBatch &hb, const std::vector<int> &pairs, LongPlan &lp);
+bool nwlong_plan(const HostBatch &hb, const std::vector<int> &pairs, int open, LongPlan &lp);
 int nwlong_launch(const DPParams &dp, const LongPlan &lp, char *region, hipStream_t st);

@@ -803,7 +803,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->nwaves = (int)qw.size();
     P->nfast = (int)refs.size();
     P->nslow = (int)slow.size();
-    if (!nwlong_plan(hb, lng, P->lp)) { delete P; return nullptr; }
+    if (!nwlong_plan(hb, lng, dp.open, P->lp)) { delete P; return nullptr; }
     P->nblocks = std::min((P->nwaves + 3) / 4, 1024);
     // DP row 0 as H-open per column (both halves), F = -inf
     const int ntop = maxw + 64 + 16;

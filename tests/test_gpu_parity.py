@@ -423,7 +423,7 @@ def test_banded_batch(mats):
 
 def test_long_nw_int32(mats):
     """Global scores of pairs too long for int16 (the multi-wave int32
-    kernel): one to three groups of 15 bands, mutated and unrelated pairs,
+    kernel): one to three groups of bands, mutated and unrelated pairs,
     through the batch and the per-call scan entry point."""
     pm, om = mats["dna"]
     prng = random.Random(17)
