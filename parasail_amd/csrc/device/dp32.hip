@@ -135,7 +135,9 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
                 F = xF;
                 if (STATS) { upS = xS; FS = xFS; }
             }
-            int diag = prevRH, diagS = prevRS;
+            // diagonal terms of the lane's first row; each next row's are
+            // taken before its registers are rewritten (no copies)
+            int t1 = prevRH + Wv[0].x, dS = STATS ? prevRS + Wv[0].y : 0;
             prevRH = up;
             prevRS = upS;
             int hv[R];
@@ -148,20 +150,21 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
                 const int Eext = E[r] - gap;
                 const bool cE = HL[r] > Eext;
                 const int En = max(HL[r], Eext);
-                const int t1 = diag + Wv[r].x;
+                const int t1n = r + 1 < R ? HL[r] + Wv[r + 1].x : 0;
                 int mEF = max(En, F);
                 if (SW) mEF = max(mEF, 0);
                 const int h = max(t1, mEF);
                 const bool cD = t1 >= mEF;
                 const bool cFE = F >= En;
                 if (STATS) {
+                    const int dSn = r + 1 < R ? HS[r] + Wv[r + 1].y : 0;
                     FS = (cF ? upS : FS) + 1;
                     ES[r] = (cE ? HS[r] : ES[r]) + 1;
-                    int hs = cD ? diagS + Wv[r].y : (cFE ? FS : ES[r]);
+                    int hs = cD ? dS : (cFE ? FS : ES[r]);
                     if (SW) hs = h > 0 ? hs : 0;
-                    diagS = HS[r];
                     HS[r] = hs;
                     upS = hs;
+                    dS = dSn;
                 }
                 if (TRACE) {
                     int code = (cF ? 32 : 64) | (cE ? 8 : 16) | (cD ? 4 : (cFE ? 2 : 1));
@@ -173,10 +176,10 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
                     }
                 }
                 E[r] = En;
-                diag = HL[r];
                 HL[r] = h - open;
                 up = HL[r];
                 hv[r] = h;
+                t1 = t1n;
             }
             botH = HL[R - 1];
             botF = F;

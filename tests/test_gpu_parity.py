@@ -440,3 +440,25 @@ def test_long_nw_int32(mats):
         assert (got["score"][i], got["end_query"][i], got["end_ref"][i]) == tuple(exp[i, :3]), i
     r = P.align("parasail_nw_scan_hip_64_32", qs[1], ts[1], 5, 2, pm)
     assert r["score"] == exp[1, 0]
+
+
+def test_long_nw_int32_table_scores(mats):
+    """The long-pair kernel with scores read from the LDS table (alphabets
+    past 8 codes, or W + open outside 4 bits): protein pairs with BLOSUM62
+    11/1 and DNA with open 20."""
+    bp, bo = mats["blosum62"]
+    dp_, do = mats["dna"]
+    prng = random.Random(18)
+    # m + n past the int16 window for BLOSUM62 11/1, so both take this kernel
+    q = rseq(prng, 20000)
+    qs = [q, q[:1200]]
+    ts = [mutate(prng, q, AA)[:14000], rseq(prng, 31000)]
+    got = P.align_batch("nw", 0, 32, qs, ts, 11, 1, bp)
+    exp, _ = O.batch(O.NW, O.SCORE, [bo.map(x) for x in qs], [bo.map(x) for x in ts], 11, 1, bo)
+    for i in range(2):
+        assert got["score"][i] == exp[i, 0], i
+    qd = rseq(prng, 17000, "ACGT")
+    td = mutate(prng, qd, "ACGT", sub=0.1, indel=0.02)[:3000]
+    got = P.align_batch("nw", 0, 32, [qd], [td], 20, 2, dp_)
+    exp, _ = O.batch(O.NW, O.SCORE, [do.map(qd)], [do.map(td)], 20, 2, do)
+    assert got["score"][0] == exp[0, 0]
