@@ -12,11 +12,15 @@
 // holds every value.
 bool dp32_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap, int m,
                    int n);
-// Trace layout code of the wavefront kernel for a query of m rows
-// (T | R << 8; 0 if the pair is outside its range) and the bytes its trace
-// takes for n columns.
-int32_t dp32_layout(int m);
-int64_t dp32_trace_bytes(int m, int n);
+// Whether one pair on its own (a per-call alignment) goes to the kernel with
+// 64-lane jobs; score-only pairs go whatever the check (extrema computed).
+bool dp32_eligible_one(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap,
+                       int m, int n);
+// Trace layout code of the wavefront kernel for a query of m rows and jobs
+// of at least min_lanes lanes (T | R << 8; 0 if the pair is outside its
+// range) and the bytes its trace takes for n columns.
+int32_t dp32_layout(int m, int min_lanes = 16);
+int64_t dp32_trace_bytes(int m, int n, int min_lanes = 16);
 
 // Jobs and waves of one pass, as a host image of the device region the
 // caller uploads (bytes long).
@@ -27,7 +31,7 @@ struct Dp32Plan {
 };
 // status[p] and troff[p] (may be NULL) are indexed by pair.
 bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t *status,
-               const int64_t *troff, Dp32Plan &P);
+               const int64_t *troff, Dp32Plan &P, int min_lanes = 16);
 int dp32_launch(const DPParams &dp, const Dp32Plan &P, const char *region, hipStream_t st);
 
 // CIGAR of every pair from its trace (row-major when tlay[p] is 0, else

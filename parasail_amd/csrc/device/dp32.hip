@@ -64,7 +64,10 @@ struct Dp32Cfg {
     phip_out_t *out;
 };
 
-template <int R, int KIND, int ALG>
+// KIND: PHIP_SCORE, PHIP_STATS or PHIP_TRACE.  EXT (score only): also the
+// exact extrema of H, E and F over the pair's cells and boundaries, for the
+// caller's saturation test (k_simple's `check`).
+template <int R, int KIND, int ALG, bool EXT>
 __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, const int2 *tab,
                                          const int lane)
 {
@@ -109,6 +112,8 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
     // end tracking: SW best (value, column, row, stats); NW/SG the cell
     // (m-1, n-1); SG the last column over rows < m-1 and the last row
     int best = kNeg, bj = 0, bi = 0, bS = 0;
+    int vmin = 0, vmax = 0;
+    const int nvalid = min(max(m - t * R, 0), R);  // rows of this lane inside the query
     int colv = kNeg, colr = 0, colS = 0;
     int rowv = kNeg, rowj = 0, rowS = 0;
     int lastv = kNeg, lastS = 0;
@@ -179,6 +184,10 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
                 HL[r] = h - open;
                 up = HL[r];
                 hv[r] = h;
+                if (EXT && r < nvalid) {
+                    vmin = min(vmin, min(h, min(En, F)));
+                    vmax = max(vmax, h);
+                }
                 t1 = t1n;
             }
             botH = HL[R - 1];
@@ -290,6 +299,15 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
             }
         }
     }
+    if (EXT) {
+        for (int k = 1; k < T; k <<= 1) {
+            vmin = min(vmin, __shfl_xor(vmin, k, T));
+            vmax = max(vmax, __shfl_xor(vmax, k, T));
+        }
+        // the DP's boundary row and column (k_simple's first row and WH)
+        if (!zero_top) vmin = min(vmin, -open - (nreal - 1) * gap);
+        if (!zero_left) vmin = min(vmin, -open - (m - 1) * gap);
+    }
     if (jv && t == 0) {
         phip_out_t o;
         memset(&o, 0, sizeof(o));
@@ -302,13 +320,15 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
             o.length = w & 0xfff;
         }
         o.status = J.status;
+        o.min_val = vmin;
+        o.max_val = vmax;
         c.out[pr] = o;
     }
 }
 
 // LDS: (size+1) x size entries {substitution + open, statistics increment},
 // the last row for query rows past the end.
-template <int KIND, int ALG>
+template <int KIND, int ALG, bool EXT>
 __global__ void __launch_bounds__(256) k_dp32(Dp32Cfg c)
 {
     extern __shared__ int2 tab[];
@@ -328,11 +348,11 @@ __global__ void __launch_bounds__(256) k_dp32(Dp32Cfg c)
     const Dp32Wave wd = c.waves[w];
     const int lane = threadIdx.x & 63;
     switch (wd.R) {
-    case 4: dp32_run<4, KIND, ALG>(c, wd, tab, lane); break;
-    case 8: dp32_run<8, KIND, ALG>(c, wd, tab, lane); break;
-    case 12: dp32_run<12, KIND, ALG>(c, wd, tab, lane); break;
-    case 16: dp32_run<16, KIND, ALG>(c, wd, tab, lane); break;
-    default: dp32_run<20, KIND, ALG>(c, wd, tab, lane); break;
+    case 4: dp32_run<4, KIND, ALG, EXT>(c, wd, tab, lane); break;
+    case 8: dp32_run<8, KIND, ALG, EXT>(c, wd, tab, lane); break;
+    case 12: dp32_run<12, KIND, ALG, EXT>(c, wd, tab, lane); break;
+    case 16: dp32_run<16, KIND, ALG, EXT>(c, wd, tab, lane); break;
+    default: dp32_run<20, KIND, ALG, EXT>(c, wd, tab, lane); break;
     }
 }
 
@@ -459,9 +479,10 @@ size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // lanes and rows per lane of a job: the fewest lanes whose band of at most
 // 20 rows per lane covers the query, rows rounded up to a multiple of 4
-bool shape(int m, int &T, int &R)
+bool shape(int m, int &T, int &R, int min_lanes = 16)
 {
     for (int tt : {16, 32, 64}) {
+        if (tt < min_lanes) continue;
         if (m <= tt * 20) {
             T = tt;
             R = std::max(4, ((m + tt - 1) / tt + 3) & ~3);
@@ -492,22 +513,32 @@ bool dp32_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int op
     return lo >= neg && hi <= pos;
 }
 
-int32_t dp32_layout(int m)
+bool dp32_eligible_one(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap,
+                       int m, int n)
+{
+    if (mx.is_pssm || mx.size > 32) return false;
+    int T, R;
+    if (!shape(m, T, R, 64)) return false;
+    if (kind == PHIP_SCORE) return true;  // extrema computed when check is set
+    return dp32_eligible(mx, alg, kind, check, open, gap, m, n);
+}
+
+int32_t dp32_layout(int m, int min_lanes)
 {
     int T = 0, R = 0;
-    if (!shape(m, T, R)) return 0;
+    if (!shape(m, T, R, min_lanes)) return 0;
     return T | (R << 8);
 }
 
-int64_t dp32_trace_bytes(int m, int n)
+int64_t dp32_trace_bytes(int m, int n, int min_lanes)
 {
     int T = 0, R = 0;
-    if (!shape(m, T, R)) return (int64_t)m * n;
+    if (!shape(m, T, R, min_lanes)) return (int64_t)m * n;
     return ((int64_t)n + T + 1) * T * R;
 }
 
 bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t *status,
-               const int64_t *troff, Dp32Plan &P)
+               const int64_t *troff, Dp32Plan &P, int min_lanes)
 {
     P = Dp32Plan();
     if (pairs.empty()) return true;
@@ -516,7 +547,7 @@ bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t
     keys.reserve(pairs.size());
     for (int p : pairs) {
         Key k;
-        if (!shape(hb.qlen[p], k.T, k.R)) return false;
+        if (!shape(hb.qlen[p], k.T, k.R, min_lanes)) return false;
         k.n = hb.tlen[p];
         k.pair = p;
         keys.push_back(k);
@@ -591,11 +622,18 @@ int dp32_launch(const DPParams &dp, const Dp32Plan &P, const char *region, hipSt
     const size_t lds = (size_t)(dp.msize + 1) * dp.msize * 8;
     const dim3 grid((P.nwaves + 3) / 4), block(256);
     typedef void (*Fn)(Dp32Cfg);
-    static const Fn fns[2][3] = {
-        {k_dp32<PHIP_STATS, PHIP_SW>, k_dp32<PHIP_STATS, PHIP_NW>, k_dp32<PHIP_STATS, PHIP_SG>},
-        {k_dp32<PHIP_TRACE, PHIP_SW>, k_dp32<PHIP_TRACE, PHIP_NW>, k_dp32<PHIP_TRACE, PHIP_SG>}};
-    if (dp.alg < 0 || dp.alg > 2) return -1;
-    hipLaunchKernelGGL(fns[dp.kind == PHIP_TRACE][dp.alg], grid, block, lds, st, c);
+    static const Fn fns[4][3] = {
+        {k_dp32<PHIP_SCORE, PHIP_SW, false>, k_dp32<PHIP_SCORE, PHIP_NW, false>,
+         k_dp32<PHIP_SCORE, PHIP_SG, false>},
+        {k_dp32<PHIP_SCORE, PHIP_SW, true>, k_dp32<PHIP_SCORE, PHIP_NW, true>,
+         k_dp32<PHIP_SCORE, PHIP_SG, true>},
+        {k_dp32<PHIP_STATS, PHIP_SW, false>, k_dp32<PHIP_STATS, PHIP_NW, false>,
+         k_dp32<PHIP_STATS, PHIP_SG, false>},
+        {k_dp32<PHIP_TRACE, PHIP_SW, false>, k_dp32<PHIP_TRACE, PHIP_NW, false>,
+         k_dp32<PHIP_TRACE, PHIP_SG, false>}};
+    if (dp.alg < 0 || dp.alg > 2 || dp.kind < 0 || dp.kind > 2) return -1;
+    const int row = dp.kind == PHIP_SCORE ? (dp.check ? 1 : 0) : dp.kind + 1;
+    hipLaunchKernelGGL(fns[row][dp.alg], grid, block, lds, st, c);
     HIP_OK(hipGetLastError());
     return 0;
 }

@@ -440,6 +440,130 @@ extern "C" int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **trace
     return rc;
 }
 
+// ---------------------------------------------------------------- one pair
+
+namespace {
+
+// pinned staging and its device mirror for per-call alignments (kept for
+// the process: every call reuses them)
+char *g_one_h = nullptr, *g_one_d = nullptr;
+size_t g_one_cap = 0;
+
+int one_reserve(size_t bytes)
+{
+    if (bytes <= g_one_cap) return 0;
+    const size_t cap = std::max(bytes + (bytes >> 1), (size_t)1 << 20);
+    if (g_one_h) (void)hipHostFree(g_one_h);
+    if (g_one_d) (void)hipFree(g_one_d);
+    g_one_h = g_one_d = nullptr;
+    g_one_cap = 0;
+    HIP_OK(hipHostMalloc((void **)&g_one_h, cap, hipHostMallocDefault));
+    HIP_OK(hipMalloc((void **)&g_one_d, cap));
+    g_one_cap = cap;
+    return 0;
+}
+
+}  // namespace
+
+// One pair through the int32 lane-chain kernel on a whole wave (64 lanes,
+// the shortest chain for one pair): one upload from pinned memory, one
+// launch, one download.  Pairs outside its range take phip_align.
+extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **traces)
+{
+    if (phip_init() != 0) return -1;
+    if (!b || !out || b->npairs != 1 || !b->q || (b->kind == PHIP_TRACE && !traces))
+        return phip_align(b, out, traces);
+    const int m = b->qlen[0], n = b->tlen[0];
+    if (m <= 0 || n <= 0 ||
+        !dp32_eligible_one(b->matrix, b->alg, b->kind, b->want_sat_check, b->open, b->gap, m, n))
+        return phip_align(b, out, traces);
+    const int64_t zero = 0;
+    HostBatch hb;
+    hb.np = 1;
+    hb.qlen = &m;
+    hb.tlen = &n;
+    hb.qoff = &zero;
+    hb.toff = &zero;
+    hb.mx = &b->matrix;
+    const std::vector<int> pairs(1, 0);
+    const int32_t status = PHIP_ST_W32;
+    Dp32Plan P;
+    if (!dp32_plan(hb, pairs, &status, &zero, P, 64)) return phip_align(b, out, traces);
+    const int msize = b->matrix.size;
+    size_t off = 0;
+    auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
+    const size_t o_plan = slot(P.bytes), o_out = slot(sizeof(phip_out_t)), o_qo = slot(8),
+                 o_to = slot(8), o_ql = slot(4), o_tl = slot(4),
+                 o_mat = slot((size_t)msize * msize * 4), o_q = slot((size_t)m),
+                 o_t = slot((size_t)n);
+    const size_t up_end = off;
+    const int64_t trbytes = b->kind == PHIP_TRACE ? dp32_trace_bytes(m, n, 64) : 0;
+    const size_t o_tr = slot((size_t)trbytes);
+    pthread_mutex_lock(&g_lock);
+    int rc = -1;
+    do {
+        if (one_reserve(off) != 0) break;
+        char *h = g_one_h, *d = g_one_d;
+        memcpy(h + o_plan, P.image.data(), P.bytes);
+        memset(h + o_qo, 0, 8);
+        memset(h + o_to, 0, 8);
+        memcpy(h + o_ql, &m, 4);
+        memcpy(h + o_tl, &n, 4);
+        memcpy(h + o_mat, b->matrix.values, (size_t)msize * msize * 4);
+        memcpy(h + o_q, b->q, (size_t)m);
+        memcpy(h + o_t, b->t, (size_t)n);
+        if (hipMemcpyAsync(d, h, up_end, hipMemcpyHostToDevice, g_stream) != hipSuccess) {
+            phip_set_error("phip_align_one: upload failed");
+            break;
+        }
+        DPParams dp;
+        memset(&dp, 0, sizeof(dp));
+        dp.alg = b->alg;
+        dp.kind = b->kind;
+        dp.sg_flags = b->alg == PHIP_SG ? b->sg_flags : 0;
+        dp.open = b->open;
+        dp.gap = b->gap;
+        dp.check = b->kind == PHIP_SCORE ? b->want_sat_check : 0;
+        dp.matrix = (const int *)(d + o_mat);
+        dp.msize = msize;
+        dp.q = (const uint8_t *)(d + o_q);
+        dp.qoff = (const int64_t *)(d + o_qo);
+        dp.qlen = (const int32_t *)(d + o_ql);
+        dp.t = (const uint8_t *)(d + o_t);
+        dp.toff = (const int64_t *)(d + o_to);
+        dp.tlen = (const int32_t *)(d + o_tl);
+        dp.npairs = 1;
+        dp.out = (phip_out_t *)(d + o_out);
+        dp.trace = (int8_t *)(d + o_tr);
+        if (dp32_launch(dp, P, d + o_plan, g_stream) != 0) break;
+        bool ok = hipMemcpyAsync(h + o_out, d + o_out, sizeof(phip_out_t), hipMemcpyDeviceToHost,
+                                 g_stream) == hipSuccess;
+        if (trbytes)
+            ok &= hipMemcpyAsync(h + o_tr, d + o_tr, (size_t)trbytes, hipMemcpyDeviceToHost,
+                                 g_stream) == hipSuccess;
+        ok &= hipStreamSynchronize(g_stream) == hipSuccess;
+        if (!ok) {
+            phip_set_error("phip_align_one: download failed");
+            break;
+        }
+        memcpy(out, h + o_out, sizeof(phip_out_t));
+        if (trbytes) {
+            // wavefront order (row t*R+r, column j at ((j+t)*T + t)*R + r)
+            // to row-major
+            const int lay = dp32_layout(m, 64), T = lay & 0xff, R = lay >> 8;
+            const int8_t *w = (const int8_t *)(h + o_tr);
+            for (int row = 0; row < m; ++row) {
+                const int t = row / R, rr = row - t * R;
+                for (int j = 0; j < n; ++j)
+                    traces[0][(size_t)row * n + j] = w[((size_t)(j + t) * T + t) * R + rr];
+            }
+        }
+        rc = 0;
+    } while (0);
+    pthread_mutex_unlock(&g_lock);
+    return rc;
+}
+
 extern "C" int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, const uint8_t *traw,
                                  phip_out_t *out, phip_cigars_t *cig)
 {

@@ -149,7 +149,7 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
         tr = (int8_t *)malloc((size_t)m * s2Len);
         if (!tr) { free(q); free(t); return NULL; }
     }
-    int rc = phip_align(&b, &o, kind == PHIP_TRACE ? &tr : NULL);
+    int rc = phip_align_one(&b, &o, kind == PHIP_TRACE ? &tr : NULL);
     free(q);
     free(t);
     if (rc != 0) {

@@ -75,6 +75,10 @@ const char *phip_last_error(void);
  * (qlen*tlen bytes, the LOC_NOVEC layout of cigar.c:156).  0 on success. */
 int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **traces);
 
+/* One pair (npairs == 1) with the lowest latency: one upload, one launch,
+ * one download when the pair fits the 64-lane kernel, else phip_align. */
+int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **traces);
+
 /* Traceback of every pair and its CIGAR, both on the device (kind must be
  * PHIP_TRACE).  qraw/traw: the residues as given, upper-cased, at the
  * batch's offsets (the walk compares residues, cigar_template.c:104-118).
