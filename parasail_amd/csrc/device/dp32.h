@@ -30,8 +30,9 @@ struct Dp32Plan {
     int njobs = 0, nwaves = 0;
 };
 // status[p] and troff[p] (may be NULL) are indexed by pair.
+// fine: rows per lane from a finer set (no traceback)
 bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t *status,
-               const int64_t *troff, Dp32Plan &P, int min_lanes = 16);
+               const int64_t *troff, Dp32Plan &P, int min_lanes = 16, bool fine = false);
 int dp32_launch(const DPParams &dp, const Dp32Plan &P, const char *region, hipStream_t st);
 
 // CIGAR of every pair from its trace (row-major when tlay[p] is 0, else

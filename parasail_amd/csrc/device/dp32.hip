@@ -352,7 +352,19 @@ __global__ void __launch_bounds__(256) k_dp32(Dp32Cfg c)
     case 8: dp32_run<8, KIND, ALG, EXT>(c, wd, tab, lane); break;
     case 12: dp32_run<12, KIND, ALG, EXT>(c, wd, tab, lane); break;
     case 16: dp32_run<16, KIND, ALG, EXT>(c, wd, tab, lane); break;
-    default: dp32_run<20, KIND, ALG, EXT>(c, wd, tab, lane); break;
+    case 20: dp32_run<20, KIND, ALG, EXT>(c, wd, tab, lane); break;
+    default:
+        // the finer shapes of per-call alignments (no traceback)
+        if constexpr (KIND != PHIP_TRACE) {
+            switch (wd.R) {
+            case 2: dp32_run<2, KIND, ALG, EXT>(c, wd, tab, lane); break;
+            case 3: dp32_run<3, KIND, ALG, EXT>(c, wd, tab, lane); break;
+            case 5: dp32_run<5, KIND, ALG, EXT>(c, wd, tab, lane); break;
+            case 6: dp32_run<6, KIND, ALG, EXT>(c, wd, tab, lane); break;
+            default: dp32_run<10, KIND, ALG, EXT>(c, wd, tab, lane); break;
+            }
+        }
+        break;
     }
 }
 
@@ -479,13 +491,20 @@ size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // lanes and rows per lane of a job: the fewest lanes whose band of at most
 // 20 rows per lane covers the query, rows rounded up to a multiple of 4
-bool shape(int m, int &T, int &R, int min_lanes = 16)
+// fine: any R of kFineRs (no traceback, whose stores take 4 rows at a time)
+const int kFineRs[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20};
+
+bool shape(int m, int &T, int &R, int min_lanes = 16, bool fine = false)
 {
     for (int tt : {16, 32, 64}) {
         if (tt < min_lanes) continue;
         if (m <= tt * 20) {
             T = tt;
-            R = std::max(4, ((m + tt - 1) / tt + 3) & ~3);
+            const int need = (m + tt - 1) / tt;
+            R = std::max(4, (need + 3) & ~3);
+            if (fine)
+                for (int r : kFineRs)
+                    if (r >= need) { R = r; break; }
             return true;
         }
     }
@@ -518,7 +537,7 @@ bool dp32_eligible_one(const phip_matrix_t &mx, int alg, int kind, int check, in
 {
     if (mx.is_pssm || mx.size > 32) return false;
     int T, R;
-    if (!shape(m, T, R, 64)) return false;
+    if (!shape(m, T, R, 64, kind != PHIP_TRACE)) return false;
     if (kind == PHIP_SCORE) return true;  // extrema computed when check is set
     return dp32_eligible(mx, alg, kind, check, open, gap, m, n);
 }
@@ -538,7 +557,7 @@ int64_t dp32_trace_bytes(int m, int n, int min_lanes)
 }
 
 bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t *status,
-               const int64_t *troff, Dp32Plan &P, int min_lanes)
+               const int64_t *troff, Dp32Plan &P, int min_lanes, bool fine)
 {
     P = Dp32Plan();
     if (pairs.empty()) return true;
@@ -547,7 +566,7 @@ bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t
     keys.reserve(pairs.size());
     for (int p : pairs) {
         Key k;
-        if (!shape(hb.qlen[p], k.T, k.R, min_lanes)) return false;
+        if (!shape(hb.qlen[p], k.T, k.R, min_lanes, fine)) return false;
         k.n = hb.tlen[p];
         k.pair = p;
         keys.push_back(k);

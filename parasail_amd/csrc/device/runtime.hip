@@ -488,7 +488,8 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
     const std::vector<int> pairs(1, 0);
     const int32_t status = PHIP_ST_W32;
     Dp32Plan P;
-    if (!dp32_plan(hb, pairs, &status, &zero, P, 64)) return phip_align(b, out, traces);
+    if (!dp32_plan(hb, pairs, &status, &zero, P, 64, b->kind != PHIP_TRACE))
+        return phip_align(b, out, traces);
     const int msize = b->matrix.size;
     size_t off = 0;
     auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
