@@ -59,9 +59,16 @@ struct BandCfg {
 // at -inf; their values only ever reach the band as the left neighbour of
 // its last slot, where any value <= -inf/2 loses the max to a real path just
 // as the reference's -inf does.
+//
+// Several waves of one workgroup may share a pair: lane g = 64 w + t of an
+// L = 64 NWV lane chain.  Inside a wave the exchanges above are DPP moves;
+// across waves they go through LDS: lane 0 of wave w publishes its first
+// slot (barrier A, mid-step) for lane 63 of wave w-1, and lane 63 of wave w
+// its bottom row (barrier B, end of step) for lane 0 of wave w+1.
 template <int R, bool LTAB>
 __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, const int pr,
-                                           const int t)
+                                           const int w, const int t, const int NWV, int2 *lx,
+                                           int2 *ux)
 {
     const int m = c.qlen[pr], n = c.tlen[pr];
     const uint8_t *s1 = c.q + c.qoff[pr];
@@ -70,7 +77,8 @@ __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, con
     const int low = (m > n) ? n - m - k : m - n - k;
     const int colLen = k - low + 1;
     const int colOff0 = (m > n) ? -k : low;
-    const int tR = t * R;
+    const int L = 64 * NWV, g = 64 * w + t;  // lanes of the chain; this lane
+    const int tR = g * R;
     const int *mrow = c.mat;
 
     // the reference's initial arrays (src/nw_banded.c:110-138); qm holds
@@ -88,8 +96,8 @@ __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, con
         Eb[r] = e;
         cap[r] = i < colLen ? INT32_MAX : BNEG;
         // the lane's codes shift one row per step from step 0 on, when its
-        // column is -t: start from that column's rows
-        const int pos = colOff0 - t + i;
+        // column is -g: start from that column's rows
+        const int pos = colOff0 - g + i;
         qm[r] = (int)s1[pos < 0 ? 0 : (pos >= m ? m - 1 : pos)] * msize;
     }
     // clamped, unconditional loads: the residue of a row outside the query
@@ -99,18 +107,28 @@ __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, con
     int botH = BNEG, botF = BNEG;
     // codes fetched two steps ahead: the target residue of the lane's column
     // and the query residue entering its last slot at the next column
-    int tq0 = ldt(-t), tq1 = ldt(1 - t);
-    int nq0 = ldq(colOff0 - t + tR + R), nq1 = ldq(colOff0 + 1 - t + tR + R);
+    int tq0 = ldt(-g), tq1 = ldt(1 - g);
+    int nq0 = ldq(colOff0 - g + tR + R), nq1 = ldq(colOff0 + 1 - g + tR + R);
 
     auto step = [&](auto fastc, const int s, int &tq, int &nq) __attribute__((always_inline)) {
         constexpr bool FAST = decltype(fastc)::value;
-        const int j = s - t;
+        const int j = s - g;
         const bool act = j >= 0 && j < n;
         const int xH = shr1(botH), xF = shr1(botF);
-        int Hup = t == 0 ? BNEG : xH;  // slot -1 of lane 0 is the -inf edge
-        int F = t == 0 ? BNEG : xF;
+        // slot -1 of lane 0 of the chain is the -inf edge; lane 0 of a later
+        // wave takes the bottom row the wave above published last step
+        int Hup = xH, F = xF;
+        if (t == 0) {
+            Hup = BNEG;
+            F = BNEG;
+            if (w > 0) {
+                const int2 u = ux[w - 1];
+                Hup = u.x;
+                F = u.y;
+            }
+        }
         const int colOff = colOff0 + j;
-        if (!FAST && colOff0 + s - 63 < 0) {
+        if (!FAST && colOff0 + s - (L - 1) < 0) {
             // some lane's column still has rows above the query: the slot
             // just above row 0 holds that column's top boundary
             const int ib = -colOff - 1 - tR;
@@ -151,14 +169,30 @@ __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, con
             Hup = Hb[r];
         };
         slot(0, Hb[1], Eb[1]);
-        // lane t+1's first slot, just computed for column j-1, is the left
+        if (NWV > 1) {
+            if (t == 0 && w > 0) lx[w] = make_int2(Hb[0], Eb[0]);
+            __syncthreads();  // A
+        }
+        // lane g+1's first slot, just computed for column j-1, is the left
         // neighbour of this lane's last slot
         int xHn = shl1(Hb[0]), xEn = shl1(Eb[0]);
-        if (t == 63) { xHn = BNEG; xEn = BNEG; }
+        if (t == 63) {
+            xHn = BNEG;
+            xEn = BNEG;
+            if (w + 1 < NWV) {
+                const int2 v = lx[w + 1];
+                xHn = v.x;
+                xEn = v.y;
+            }
+        }
 #pragma unroll
         for (int r = 1; r < R; ++r) slot(r, r + 1 < R ? Hb[r + 1] : xHn, r + 1 < R ? Eb[r + 1] : xEn);
         botH = Hup;
         botF = F;
+        if (NWV > 1) {
+            if (t == 63 && w + 1 < NWV) ux[w] = make_int2(botH, botF);
+            __syncthreads();  // B
+        }
         // next column: every slot moves one row down the query
 #pragma unroll
         for (int r = 0; r < R - 1; ++r) qm[r] = qm[r + 1];
@@ -171,9 +205,9 @@ __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, con
     // steps where every lane is on a column of the target and every band
     // slot on a row of the query, rounded inwards to even steps
     const int tl = (colLen - 1) / R;
-    const int sa = (max(63, -colOff0) + 1) & ~1;
+    const int sa = (max(L - 1, -colOff0) + 1) & ~1;
     const int sb = max(sa, min(n, m + tl + 1 - colOff0 - colLen) & ~1);
-    const int steps = n + 63;
+    const int steps = n + L - 1;
     int s = 0;
     for (; s < sa; s += 2) {
         step(slow, s, tq0, nq0);
@@ -202,28 +236,38 @@ __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, con
     }
 }
 
-// One wave per pair; c.rsel gives each pair's slots per lane (at most RMAX).
+// One workgroup per pair (blockDim/64 waves, one lane chain); c.rsel gives
+// each pair's slots per lane (at most RMAX).  LDS: the square matrix
+// (LTAB), then the two 8-entry exchange rows of band_chain.
 template <int RMAX, bool LTAB>
-__global__ void __launch_bounds__(256) k_banded_chain(BandCfg c)
+__global__ void __launch_bounds__(512) k_banded_chain(BandCfg c)
 {
     extern __shared__ int tab[];
     if (LTAB)
         for (int x = threadIdx.x; x < c.msize * c.msize; x += blockDim.x) tab[x] = c.mat[x];
+    int2 *const lx = (int2 *)(tab + (LTAB ? (c.msize * c.msize + 1) & ~1 : 0));
+    int2 *const ux = lx + 8;
     __syncthreads();
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= c.npairs) return;  // whole wave
-    const int t = threadIdx.x & 63;
-    const int pr = c.pairs[w];
-    switch (c.rsel[w]) {
-    case 2: band_chain<2, LTAB>(c, tab, pr, t); break;
-    case 4: band_chain<4, LTAB>(c, tab, pr, t); break;
-    case 8: band_chain<8, LTAB>(c, tab, pr, t); break;
-    case 12: if (RMAX >= 12) band_chain<(RMAX >= 12 ? 12 : 2), LTAB>(c, tab, pr, t); break;
-    case 16: if (RMAX >= 16) band_chain<(RMAX >= 16 ? 16 : 2), LTAB>(c, tab, pr, t); break;
-    case 24: if (RMAX >= 24) band_chain<(RMAX >= 24 ? 24 : 2), LTAB>(c, tab, pr, t); break;
-    case 32: if (RMAX >= 32) band_chain<(RMAX >= 32 ? 32 : 2), LTAB>(c, tab, pr, t); break;
+    if (blockIdx.x >= (unsigned)c.npairs) return;  // whole block
+    const int NWV = blockDim.x >> 6, w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int pr = c.pairs[blockIdx.x];
+#define BAND_R(RR) band_chain<(RMAX >= RR ? RR : 2), LTAB>(c, tab, pr, w, t, NWV, lx, ux)
+    switch (c.rsel[blockIdx.x]) {
+    case 2: BAND_R(2); break;
+    case 3: BAND_R(3); break;
+    case 4: BAND_R(4); break;
+    case 5: if (RMAX >= 5) BAND_R(5); break;
+    case 6: if (RMAX >= 6) BAND_R(6); break;
+    case 8: if (RMAX >= 8) BAND_R(8); break;
+    case 10: if (RMAX >= 10) BAND_R(10); break;
+    case 12: if (RMAX >= 12) BAND_R(12); break;
+    case 16: if (RMAX >= 16) BAND_R(16); break;
+    case 20: if (RMAX >= 20) BAND_R(20); break;
+    case 24: if (RMAX >= 24) BAND_R(24); break;
+    case 32: if (RMAX >= 32) BAND_R(32); break;
     default: break;
     }
+#undef BAND_R
 }
 
 // Single-lane fallback for bands wider than 64 x 32 slots (one pair per
@@ -287,6 +331,7 @@ typedef void (*BFn)(BandCfg);
 template <bool LTAB>
 BFn chain_kernel_t(int rmax)
 {
+    if (rmax <= 6) return k_banded_chain<6, LTAB>;
     if (rmax <= 8) return k_banded_chain<8, LTAB>;
     if (rmax <= 16) return k_banded_chain<16, LTAB>;
     return k_banded_chain<32, LTAB>;
@@ -297,7 +342,7 @@ BFn chain_kernel(int rmax, bool ltab)
     return ltab ? chain_kernel_t<true>(rmax) : chain_kernel_t<false>(rmax);
 }
 
-const int kBandRs[] = {2, 4, 8, 12, 16, 24, 32};
+const int kBandRs[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
 
 size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -320,9 +365,11 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         phip_set_error("phip_nw_banded_batch: bad arguments");
         return -1;
     }
-    // lane chains for bands up to 64 x 32 slots (bucketed by slots per
-    // lane), the single-lane loop for wider ones; gaps large enough to
-    // overflow int32 over a band fall back too
+    // lane chains of 64 x nwv lanes (nwv waves of a workgroup per pair:
+    // more for few pairs, which otherwise leave most SIMDs idle) for bands
+    // up to 64 x nwv x 32 slots, the single-lane loop for wider ones; gaps
+    // large enough to overflow int32 over a band fall back too
+    const int nwv = np <= 512 ? 4 : (np <= 1024 ? 2 : 1);
     std::vector<int32_t> chain, rsel(np, 0), wide;
     std::vector<int64_t> work(np, 0);
     int rmax = 0;
@@ -340,13 +387,13 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         maxcol = std::max(maxcol, colLen);
         int R = 0;
         for (int r : kBandRs)
-            if (colLen <= 64 * r) { R = r; break; }
+            if (colLen <= 64 * nwv * r) { R = r; break; }
         if (R == 0 || b->open > (1 << 20) || b->gap > (1 << 20)) {
             wide.push_back(i);
         } else {
             chain.push_back(i);
             rsel[i] = R;
-            work[i] = (int64_t)(n + 64) * (R + 4);
+            work[i] = (int64_t)(n + 64 * nwv) * (R + 4);
             rmax = std::max(rmax, R);
         }
     }
@@ -407,13 +454,13 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         }
         BOK(hipEventRecord(phip_event(0), st));
         const bool ltab = !b->matrix.is_pssm && b->matrix.size <= 64;
-        const size_t lds = ltab ? (size_t)b->matrix.size * b->matrix.size * 4 : 0;
+        const size_t lds = (ltab ? (size_t)((b->matrix.size * b->matrix.size + 1) & ~1) * 4 : 0) + 16 * 8;
         if (!chain.empty()) {
             BandCfg cc = c;
             cc.pairs = dpairs;
             cc.rsel = drsel;
             cc.npairs = (int)chain.size();
-            hipLaunchKernelGGL(chain_kernel(rmax, ltab), dim3((cc.npairs + 3) / 4), dim3(256), lds, st, cc);
+            hipLaunchKernelGGL(chain_kernel(rmax, ltab), dim3(cc.npairs), dim3(64 * nwv), lds, st, cc);
             BOK(hipGetLastError());
         }
         if (!wide.empty()) {
