@@ -6,7 +6,7 @@
 // band w of the group, its 64 lanes forming the wavefront chain of the int16
 // kernels (lane t holds R rows and computes column s - t at step s; the row
 // above comes from lane t-1 by DPP), here in int32 because long global scores
-// leave int16.  The LW bands run as a pipeline: band w starts L = 2*LK steps
+// leave int16.  The LW bands run as a pipeline: band w starts LL = LK + 66 steps
 // after band w-1 and takes the bottom row of band w-1 from an LDS ring; all
 // waves meet at a barrier every LK steps, and every ring value a wave reads
 // was written at least one barrier earlier.
@@ -32,9 +32,18 @@
 namespace {
 
 constexpr int LW = 16;     // waves (bands in flight) per workgroup: 4 per SIMD
-constexpr int LK = 128;    // steps between barriers
-constexpr int LL = 2 * LK; // lag between consecutive bands
+constexpr int LK = 32;     // steps between barriers
+constexpr int LL = LK + 66; // lag between consecutive bands
 constexpr int LRING = 512; // ring slots per band boundary (power of two)
+// In the steps [P, P+LK) between two barriers band w+1's lane 0 reads (two
+// steps ahead) columns up to P + LK + 1 - LL*(w+1), while band w's lane 63
+// had written, before the barrier at P, the columns up to P - 64 - LL*w:
+// every read is of a value written at least one barrier earlier iff
+// LL >= LK + 65.  The writer runs at most LK + LL - 64 columns ahead of the
+// oldest column the reader still holds, which the ring must cover.
+static_assert(LL >= LK + 65, "band lag too short for the barrier spacing");
+static_assert(LK + LL - 64 < LRING, "ring too small for the band lag");
+static_assert(LK % 2 == 0, "steps run in pairs");
 constexpr int LNEG = -(1 << 29);
 constexpr int LWNEG = -(1 << 24);  // substitution value of the pad rows
 constexpr int kLongR = 16;
