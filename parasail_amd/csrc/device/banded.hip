@@ -41,6 +41,7 @@ struct BandCfg {
     const int *mat;        // device, row-major (square: size x size; PSSM: m x size)
     int msize, is_pssm;
     int open, gap, k;
+    int lring;             // bottom-row slots per wave (k_banded_rows)
     const int32_t *pairs;  // pairs of this launch
     const int32_t *rsel;   // slots per lane of each (k_banded_chain)
     int npairs;
@@ -171,7 +172,7 @@ __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, con
         slot(0, Hb[1], Eb[1]);
         if (NWV > 1) {
             if (t == 0 && w > 0) lx[w] = make_int2(Hb[0], Eb[0]);
-            __syncthreads();  // A
+            lds_barrier();  // A
         }
         // lane g+1's first slot, just computed for column j-1, is the left
         // neighbour of this lane's last slot
@@ -191,7 +192,7 @@ __device__ __forceinline__ void band_chain(const BandCfg &c, const int *tab, con
         botF = F;
         if (NWV > 1) {
             if (t == 63 && w + 1 < NWV) ux[w] = make_int2(botH, botF);
-            __syncthreads();  // B
+            lds_barrier();  // B
         }
         // next column: every slot moves one row down the query
 #pragma unroll
@@ -321,6 +322,281 @@ __global__ void __launch_bounds__(64) k_banded_one(BandCfg c, int *scratch, int6
     c.out[pr] = o;
 }
 
+// ------------------------------------------------------------ row bands
+//
+// Long pairs: the same band in (row, column) coordinates.  Row r of the
+// query lies in the band for the columns [r + c0, r + c0 + colLen - 1],
+// c0 = 1 - colOff0 - colLen, and the reference's band (stale slots and -inf
+// edges included) is the global recurrence of src/nw_banded.c:143-168 over
+// those cells with every cell outside the band at -inf, the top row
+// H(-1, j) = -open - j*gap where row -1 lies in the band, H(-1, -1) = 0, and
+// the left column H(r, -1) = -open - r*gap for the rows in the band at
+// column -1.  Here every dependency points down or right, so the query's
+// rows are cut into bands of 64 R rows, lane t of a wave holds R rows of
+// one band and computes column a + s - t at step s (a: the band's first
+// column), the row above coming from lane t-1 by DPP and, for lane 0, from
+// the band above through LDS.  A band's lanes meet their rows' colLen band
+// columns in turn (R + 1 steps apart), so a band takes about
+// 64 R + colLen + 64 steps; cells outside the band are masked to -inf.
+//
+// The NWV waves of a workgroup take the bands of one pair in turn: band b
+// runs on wave b % NWV, starting G >= RB_L steps (on the workgroup's common
+// step count) after band b-1 reached the same column, and the waves meet at
+// a barrier every RB_K steps.  Wave w's lane 63 stores its bottom row at step
+// s in slot s of the wave's LDS row (two steps per store); lane 0 of the
+// next band loads, two steps ahead and two steps per load, what was stored
+// at least G - 64 R - 68 >= RB_K steps earlier, so every value it loads was
+// stored before the last barrier.  A wave's next band starts NWV G steps
+// after its last one, at least a band's steps plus RB_K: each wave holds
+// one band per stretch between barriers, and overwrites a slot long after
+// the next band loaded it.
+constexpr int RB_K = 32;          // steps between barriers
+constexpr int RB_L = RB_K + 68;   // lag of a band behind the band above
+constexpr int RNEG = -(1 << 29);  // -inf: the host bounds every score above it
+static_assert(RB_L >= RB_K + 67, "band lag too short for the barrier spacing");
+
+// steps between consecutive row bands of 64 R = br rows taking send steps
+// each, on nwv waves (even, like every band start)
+__host__ __device__ __forceinline__ int rows_spacing(int br, int send, int nwv)
+{
+    const int q = ((send + RB_K + nwv - 1) / nwv + 1) & ~1;
+    return br + RB_L > q ? br + RB_L : q;
+}
+
+__device__ __forceinline__ int rol1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, false); }
+// v of lane t-1; lane 0 gets old
+__device__ __forceinline__ int shr1_or(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xF, 0xF, false);
+}
+
+template <int R, bool RP>
+__device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, const int pr, const int w,
+                                          const int t, const int NWV, int2 *rings)
+{
+    constexpr int BR = 64 * R;
+    const int m = c.qlen[pr], n = c.tlen[pr];
+    const uint8_t *s1 = c.q + c.qoff[pr];
+    const uint8_t *s2 = c.t + c.toff[pr];
+    const int open = c.open, gap = c.gap, k = c.k, msize = c.msize;
+    const int low = (m > n) ? n - m - k : m - n - k;
+    const int colLen = k - low + 1;
+    const int colOff0 = (m > n) ? -k : low;
+    const int c0 = 1 - colOff0 - colLen;  // first band column of row r: r + c0
+    const int lb = colOff0 + colLen - 2;  // rows <= lb lie in the band at column -1
+    const int nb = (m + BR - 1) / BR;
+    // a band's steps: lane 63 passes its last band column, then stores
+    // -inf for the R + 2 columns lane 0 of the next band still loads
+    const int send = (BR + colLen + R + 64 + 1) & ~1;
+    int2 *const ring_out = rings + w * c.lring;
+    // slot of the column lane 0 computes at step s: BR + s + 63
+    const int2 *const ring_in = rings + (w == 0 ? NWV - 1 : w - 1) * c.lring + BR + 63;
+    // steps between the starts of consecutive bands: at least the lag RB_L
+    // behind the band above, and 1/NWV of a band's steps plus RB_K, so that
+    // a wave is free again when its next band starts
+    const int G = rows_spacing(BR, send, NWV);
+    const int nph = ((nb - 1) * G + send + RB_K - 1) / RB_K;
+    const int tp = t * (R + 1);
+
+    int HL[R], E[R], qp[R];  // H - open and E of the lane's rows; their scores
+    int prevRH = RNEG, botH = RNEG, botF = RNEG, tq = 0, leftTop = RNEG, r0 = 0, a = 0;
+    // cbuf: codes of the 64 columns from lane 0's, rotated one lane a step
+    // (lane 0 holds lane 0's column); cnext: the next 64, as loaded
+    int cbuf = 0, cnext = 0;
+    int2 in0 = make_int2(RNEG, RNEG), in1 = in0;
+    auto ldc = [&](int j) -> int { return (int)s2[j < 0 ? 0 : (j >= n ? n - 1 : j)]; };
+    int b = w;
+    // loaded one band ahead, so that a band starts without waiting on HBM:
+    // the query codes of the lane's rows in the wave's next band and that
+    // band's first 128 column codes
+    int nqc[R], ncb0, ncb1;
+    auto prefetch = [&](int bn) {
+        const int rn = bn * BR + t * R;
+#pragma unroll
+        for (int i = 0; i < R; ++i) nqc[i] = (int)s1[rn + i < m ? rn + i : m - 1];
+        const int an = bn * BR + c0;
+        ncb0 = ldc(an + t);
+        ncb1 = ldc(an + 64 + t);
+    };
+    prefetch(b);
+    // lane 0's row above at step s for band 0: the top row of the DP
+    auto top = [&](int s) -> int2 {
+        const int j = a + s;
+        return make_int2(j >= 0 && j <= -1 - colOff0 ? -open - j * gap - open : RNEG, RNEG);
+    };
+    auto ld_in = [&](int s) -> int2 { return b == 0 ? top(s) : ring_in[s]; };
+
+    auto step = [&](auto slowc, const int s, const int2 in) __attribute__((always_inline)) {
+        constexpr bool SLOW = decltype(slowc)::value;
+        // lane 0 takes the new column's code and the row above from LDS;
+        // every other lane its left neighbour's, one step old
+        tq = shr1_or(cbuf, tq);
+        cbuf = rol1(cbuf);
+        const int xH = shr1_or(in.x, botH);
+        const int xF = shr1_or(in.y, botF);
+        int W[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) W[i] = RP ? (int)__builtin_amdgcn_ubfe((unsigned)qp[i], tq, 4) : tab[qp[i] + tq];
+        const int p = s - tp;  // this lane's column minus its first row's first band column
+        int t1 = prevRH + W[0];
+        prevRH = xH;
+        bool act = true;
+        if (SLOW) {
+            const int j = a + s - t;
+            if (j == 0) t1 = leftTop + W[0];
+            act = j >= 0 && j < n;
+        }
+        if (act) {
+            int up = xH, F = xF;
+            bool v = false;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                v = (unsigned)(p - i) < (unsigned)colLen;
+                if (SLOW) v = v && r0 + i < m;
+                E[i] = max(E[i] - gap, HL[i]);
+                const int t1n = i + 1 < R ? HL[i] + W[i + 1] : 0;
+                F = max(F - gap, up);
+                const int h = max(max(t1, E[i]), F);
+                HL[i] = v ? h - open : RNEG;
+                up = HL[i];
+                t1 = t1n;
+            }
+            botH = HL[R - 1];
+            botF = v ? F : RNEG;
+        }
+    };
+    auto run = [&](auto slowc, int s, const int e) __attribute__((always_inline)) {
+        constexpr bool SLOW = decltype(slowc)::value;
+        while (s < e) {
+            if ((s & 63) == 0 && s != 0) {
+                cbuf = RP ? 4 * cnext : cnext;
+                cnext = ldc(a + s + 64 + t);
+            }
+            const int ce = min(e, (s | 63) + 1);
+        for (; s < ce; s += 2) {
+            int2 nx0, nx1;
+            if (SLOW && b == 0) {
+                nx0 = top(s + 2);
+                nx1 = top(s + 3);
+            } else {
+                nx0 = ring_in[s + 2];
+                nx1 = ring_in[s + 3];
+            }
+            step(slowc, s, in0);
+            const int h0 = botH, f0 = botF;
+            step(slowc, s + 1, in1);
+            if (t == 63) *(int4 *)(ring_out + s) = make_int4(h0, f0, botH, botF);
+            in0 = nx0;
+            in1 = nx1;
+        }
+        }
+    };
+    const std::integral_constant<bool, false> fastc;
+    const std::integral_constant<bool, true> slowc;
+
+    for (int ph = 0; ph < nph; ++ph) {
+        const int P = ph * RB_K;
+        if (b < nb && b * G + send <= P) b += NWV;
+        if (b < nb) {
+            const int Tb = b * G;
+            const int s = max(0, P - Tb), e = min(send, P + RB_K - Tb);
+            if (s < e) {
+                a = b * BR + c0;
+                if (s == 0) {
+                    // the band's rows at column -1: the left column or -inf
+                    r0 = b * BR + t * R;
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        const int r = r0 + i, rc = r < m ? r : m - 1;
+                        if (RP && c.is_pssm) {
+                            // a PSSM row: its profile from HBM
+                            const int *mr = c.mat + (size_t)rc * msize;
+                            int pf = 0;
+                            for (int x = 0; x < msize; ++x) pf |= (mr[x] + open) << (4 * x);
+                            qp[i] = pf;
+                        } else {
+                            // the profile of the row's letter, or its table row
+                            qp[i] = RP ? tab[nqc[i]] : nqc[i] * msize;
+                        }
+                        HL[i] = r < m && r <= lb ? -open - r * gap - open : RNEG;
+                        E[i] = RNEG;
+                    }
+                    const int rt = r0 - 1;
+                    leftTop = rt == -1 ? -open : (rt <= lb ? -open - rt * gap - open : RNEG);
+                    botH = botF = RNEG;
+                    tq = 0;
+                    in0 = ld_in(0);
+                    in1 = ld_in(1);
+                    prevRH = t == 0 ? ld_in(-1).x : RNEG;
+                    cbuf = RP ? 4 * ncb0 : ncb0;
+                    cnext = ncb1;
+                    prefetch(b + NWV);
+                }
+                // the first bands (columns below 0, the top row, the left
+                // column) and the last (rows past the query, the score)
+                // check every cell; the others only the band's edges
+                if (a >= 1 && b < nb - 1)
+                    run(fastc, s, e);
+                else
+                    run(slowc, s, e);
+                if (e == send && b == nb - 1) {
+                    // lanes stop at column n-1: the score is H(m-1, n-1)
+                    const int rr = m - 1 - b * BR;
+                    if (t == rr / R) {
+                        int h = 0;
+#pragma unroll
+                        for (int i = 0; i < R; ++i) h = i == rr % R ? HL[i] : h;
+                        phip_out_t o;
+                        memset(&o, 0, sizeof(o));
+                        o.score = h + open;
+                        o.end_query = m - 1;
+                        o.end_ref = n - 1;
+                        o.status = PHIP_ST_W32;
+                        c.out[pr] = o;
+                    }
+                }
+            }
+        }
+        lds_barrier();
+    }
+}
+
+// One workgroup (NWV = blockDim/64 waves) per pair; c.rsel gives each
+// pair's rows per lane.  LDS: the W + open table (!RP) or the register
+// profile of each letter (RP: W + open of each target letter in 4-bit
+// fields), then NWV rows of c.lring bottom-row slots, set to -inf first.
+template <bool RP>
+__global__ void __launch_bounds__(256) k_banded_rows(BandCfg c)
+{
+    extern __shared__ int tab[];
+    const int tabn = ((RP ? c.msize : c.msize * c.msize) + 3) & ~3;
+    if (!RP) {
+        for (int x = threadIdx.x; x < c.msize * c.msize; x += blockDim.x) tab[x] = c.mat[x] + c.open;
+    } else if (!c.is_pssm) {
+        for (int x = threadIdx.x; x < c.msize; x += blockDim.x) {
+            int pf = 0;
+            for (int y = 0; y < c.msize; ++y) pf |= (c.mat[x * c.msize + y] + c.open) << (4 * y);
+            tab[x] = pf;
+        }
+    }
+    int2 *const rings = (int2 *)(tab + tabn);
+    const int NWV = blockDim.x >> 6;
+    for (int x = threadIdx.x; x < NWV * c.lring; x += blockDim.x) rings[x] = make_int2(RNEG, RNEG);
+    __syncthreads();
+    if (blockIdx.x >= (unsigned)c.npairs) return;  // whole block
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
+    const int pr = c.pairs[blockIdx.x];
+    switch (c.rsel[blockIdx.x]) {
+    case 1: band_rows<1, RP>(c, tab, pr, w, t, NWV, rings); break;
+    case 2: band_rows<2, RP>(c, tab, pr, w, t, NWV, rings); break;
+    case 3: band_rows<3, RP>(c, tab, pr, w, t, NWV, rings); break;
+    case 4: band_rows<4, RP>(c, tab, pr, w, t, NWV, rings); break;
+    case 6: band_rows<6, RP>(c, tab, pr, w, t, NWV, rings); break;
+    case 8: band_rows<8, RP>(c, tab, pr, w, t, NWV, rings); break;
+    default: break;
+    }
+}
+
 // ------------------------------------------------------------------ host
 
 namespace {
@@ -343,6 +619,41 @@ BFn chain_kernel(int rmax, bool ltab)
 }
 
 const int kBandRs[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
+
+// k_banded_rows: four waves per pair, rows per lane from this set
+constexpr int kRowsWaves = 4;
+const int kRowsRs[] = {1, 2, 3, 4, 6, 8};
+constexpr size_t kLdsMax = 160 * 1024;
+
+// steps of one row band (band_rows' send)
+int64_t rows_band_steps(int R, int64_t colLen) { return (64 * R + colLen + R + 64 + 1) & ~(int64_t)1; }
+// bottom-row slots a wave's LDS row needs: the band's steps, plus the
+// BR + 66 further slots lane 0 of the next band loads past them
+int64_t rows_slots(int R, int64_t colLen) { return rows_band_steps(R, colLen) + 64 * R + 66; }
+
+// the pair's steps on the row-band kernel, with R rows per lane
+int64_t rows_steps(int R, int64_t colLen, int64_t m)
+{
+    const int64_t send = rows_band_steps(R, colLen);
+    const int64_t nb = (m + 64 * R - 1) / (64 * R);
+    return (nb - 1) * rows_spacing(64 * R, (int)send, kRowsWaves) + send;
+}
+
+// rows per lane for an m-row query: the least steps times the measured
+// cost of a step (about 27 + 10 R instruction issues)
+int rows_pick(int64_t colLen, int64_t m)
+{
+    int best = 0;
+    int64_t best_cost = 0;
+    for (int R : kRowsRs) {
+        const int64_t cost = rows_steps(R, colLen, m) * (27 + 10 * R);
+        if (!best || cost < best_cost) {
+            best = R;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
 
 size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -370,10 +681,24 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
     // up to 64 x nwv x 32 slots, the single-lane loop for wider ones; gaps
     // large enough to overflow int32 over a band fall back too
     const int nwv = np <= 512 ? 4 : (np <= 1024 ? 2 : 1);
-    std::vector<int32_t> chain, rsel(np, 0), wide;
+    std::vector<int32_t> chain, rsel(np, 0), wide, rows;
     std::vector<int64_t> work(np, 0);
     int rmax = 0;
-    int64_t qtot = 0, ttot = 0, maxcol = 0;
+    int64_t qtot = 0, ttot = 0, maxcol = 0, rows_send = 0;  // rows_send: LDS slots per wave
+    // row bands (k_banded_rows) for pairs of >= 512 query rows whose band
+    // fits the rings, when every W + open fits a 4-bit register field
+    // (square or PSSM, <= 8 letters) or the square matrix an LDS table, and
+    // every score of the pair stays far above the kernel's -inf
+    const bool is_pssm = b->matrix.is_pssm;
+    const int msz = b->matrix.size;
+    bool rows_rp = msz <= 8;
+    int64_t wabs = 0;
+    for (int64_t x = 0; x < (int64_t)msz * b->matrix.length; ++x) {
+        const int v = b->matrix.values[x];
+        wabs = std::max<int64_t>(wabs, v < 0 ? -(int64_t)v : v);
+        if (v + b->open < 0 || v + b->open > 15) rows_rp = false;
+    }
+    const bool rows_ok = rows_rp || (!is_pssm && msz <= 64);
     for (int i = 0; i < np; ++i) {
         const int m = b->qlen[i], n = b->tlen[i];
         if (m <= 0 || n <= 0) {
@@ -385,6 +710,20 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         const int64_t low = (m > n) ? (int64_t)n - m - k : (int64_t)m - n - k;
         const int64_t colLen = k - low + 1;
         maxcol = std::max(maxcol, colLen);
+        if (rows_ok && m >= 512 && colLen <= 1 << 14 &&
+            ((int64_t)m + n + 2) * ((int64_t)b->open + b->gap + wabs) < (1 << 27)) {
+            const int R = rows_pick(colLen, m);
+            const int64_t steps = R ? rows_band_steps(R, colLen) : 0;
+            const int64_t slots = R ? std::max(rows_slots(R, colLen), rows_send) : 0;
+            const size_t tabb = (size_t)(((rows_rp ? msz : msz * msz) + 3) & ~3) * 4;
+            if (R && tabb + (size_t)kRowsWaves * slots * 8 <= kLdsMax) {
+                rows.push_back(i);
+                rsel[i] = R;
+                rows_send = slots;
+                work[i] = rows_steps(R, colLen, m) * (27 + 10 * R);
+                continue;
+            }
+        }
         int R = 0;
         for (int r : kBandRs)
             if (colLen <= 64 * nwv * r) { R = r; break; }
@@ -397,8 +736,9 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
             rmax = std::max(rmax, R);
         }
     }
-    // one launch; the longest chains start first
+    // one launch each; the longest chains start first
     std::stable_sort(chain.begin(), chain.end(), [&](int x, int y) { return work[x] > work[y]; });
+    std::stable_sort(rows.begin(), rows.end(), [&](int x, int y) { return work[x] > work[y]; });
     const size_t mbytes = (size_t)b->matrix.size * b->matrix.length * 4;
     const int64_t stride = maxcol + 2;
     size_t off = 0;
@@ -444,7 +784,10 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         int32_t *drsel = (int32_t *)(base + o_rsel);
         std::vector<int32_t> order(chain), rs;
         order.insert(order.end(), wide.begin(), wide.end());
+        order.insert(order.end(), rows.begin(), rows.end());
         for (int i : chain) rs.push_back(rsel[i]);
+        for (size_t x = 0; x < wide.size(); ++x) rs.push_back(0);
+        for (int i : rows) rs.push_back(rsel[i]);
         ok &= hipMemcpyAsync(dpairs, order.data(), order.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
         if (!rs.empty())
             ok &= hipMemcpyAsync(drsel, rs.data(), rs.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
@@ -468,6 +811,21 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
             cc.pairs = dpairs + chain.size();
             cc.npairs = (int)wide.size();
             hipLaunchKernelGGL(k_banded_one, dim3(cc.npairs), dim3(64), 0, st, cc, (int *)(base + o_scr), stride);
+            BOK(hipGetLastError());
+        }
+        if (!rows.empty()) {
+            BandCfg cc = c;
+            const size_t base_i = chain.size() + wide.size();
+            cc.pairs = dpairs + base_i;
+            cc.rsel = drsel + base_i;
+            cc.npairs = (int)rows.size();
+            const int lr = (int)rows_send;  // even: band steps and 64 R + 66 are
+            cc.lring = lr;
+            const size_t tabb = (size_t)(((rows_rp ? msz : msz * msz) + 3) & ~3) * 4;
+            const size_t lds_r = tabb + (size_t)kRowsWaves * lr * 8;
+            auto fn = rows_rp ? k_banded_rows<true> : k_banded_rows<false>;
+            BOK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r));
+            hipLaunchKernelGGL(fn, dim3(cc.npairs), dim3(64 * kRowsWaves), lds_r, st, cc);
             BOK(hipGetLastError());
         }
         BOK(hipEventRecord(phip_event(1), st));

@@ -31,6 +31,15 @@ struct DPParams {
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 
+// Workgroup barrier that orders LDS accesses only: loads from HBM still in
+// flight stay in flight (__syncthreads also waits for every one of them).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 #define HIP_OK(expr)                                                        \
     do {                                                                    \
         hipError_t _e = (expr);                                             \

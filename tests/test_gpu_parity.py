@@ -421,6 +421,47 @@ def test_banded_batch(mats):
         assert got["score"][i] == O.nw_banded(big_q[i], big_t[i], 5, 2, 256, om)[0], i
 
 
+def test_banded_rows(mats):
+    """Pairs of >= 512 query rows take the row-band kernel: bands from 11 to
+    1,800 columns (every rows-per-lane choice), the query longer and shorter
+    than the target, bands reaching past both ends of the query, scores in
+    registers (DNA 2/-3, 5/2; a PSSM of it) and in an LDS table (BLOSUM62
+    11/1; DNA with a gap extension of 4 above an open of 2)."""
+    L = P.lib()
+    prng = random.Random(21)
+    shapes = [(512, 530, 5), (700, 640, 40), (3000, 3100, 20), (2500, 2400, 200),
+              (9000, 9050, 256), (6000, 6000, 700), (4000, 3000, 300), (1500, 1400, 800),
+              (5000, 5200, 100), (2000, 2002, 450)]
+    for key, alpha, o, g in (("dna", "ACGT", 5, 2), ("blosum62", AA, 11, 1), ("dna", "ACGT", 2, 4)):
+        pm, om = mats[key]
+        qs, ts, ks = [], [], []
+        for (m, n, k) in shapes:
+            s1 = rseq(prng, m, alpha)
+            s2 = mutate(prng, s1, alpha, sub=0.1, indel=0.02)
+            s2 = (s2 + rseq(prng, max(0, n - len(s2)), alpha))[:n]
+            qs.append(s1)
+            ts.append(s2)
+            ks.append(k)
+        for k in sorted(set(ks)):
+            idx = [i for i in range(len(qs)) if ks[i] == k]
+            got = P.nw_banded_batch([qs[i] for i in idx], [ts[i] for i in idx], o, g, k, pm)
+            for x, i in enumerate(idx):
+                exp = O.nw_banded(qs[i], ts[i], o, g, k, om)
+                assert (got["score"][x], got["end_query"][x], got["end_ref"][x]) == exp[:3], \
+                    (key, o, g, len(qs[i]), len(ts[i]), k)
+    pm, om = mats["dna"]
+    for (m, n, k) in ((3000, 2950, 64), (800, 900, 300)):
+        s1 = rseq(prng, m, "ACGT")
+        s2 = mutate(prng, s1, "ACGT", sub=0.1, indel=0.02)[:n]
+        pp = L.parasail_matrix_convert_square_to_pssm(pm, s1.encode(), len(s1))
+        op = O.convert_square_to_pssm(om, s1)
+        r = L.parasail_nw_banded_hip(s1.encode(), len(s1), s2.encode(), len(s2), 5, 2, k, pp)
+        exp = O.nw_banded(s1, s2, 5, 2, k, op)
+        assert r.contents.score == exp[0], (m, n, k)
+        L.parasail_result_free(r)
+        L.parasail_matrix_free(pp)
+
+
 def test_long_nw_int32(mats):
     """Global scores of pairs too long for int16 (the multi-wave int32
     kernel): one to three groups of bands, mutated and unrelated pairs,
