@@ -154,6 +154,7 @@ struct phip_resident {
     char *dplan_mem = nullptr;
     std::vector<int32_t> slow, tlay;
     int32_t *dslow = nullptr, *dtlay = nullptr;
+    phip_out_t *hout = nullptr;  // pinned host copy of the results (NULL: none)
 };
 
 typedef struct phip_resident phip_resident_t;
@@ -224,6 +225,13 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
         delete r;
         return nullptr;
     }
+    // results come back through pinned memory (a pageable copy of a large
+    // batch's results costs more than some of its kernels)
+    if (hipHostMalloc((void **)&r->hout, (size_t)np * sizeof(phip_out_t), hipHostMallocDefault) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        r->hout = nullptr;
+    }
     return r;
 }
 
@@ -233,6 +241,7 @@ static void resident_free(phip_resident_t *r)
     for (WavePlan *p : r->plans) wave_plan_free(p);
     if (r->dplan_mem) dev_free(r->dplan_mem);
     dev_free(r->base);
+    if (r->hout) (void)hipHostFree(r->hout);
     delete r;
 }
 
@@ -648,11 +657,18 @@ extern "C" void *phip_resident_create(const phip_batch_t *b)
     return r;
 }
 
+extern "C" const phip_out_t *phip_resident_results(void *handle)
+{
+    return handle ? ((phip_resident_t *)handle)->hout : nullptr;
+}
+
 extern "C" int phip_resident_align(void *handle, int alg, int sg_flags, int open, int gap,
                                    phip_out_t *out)
 {
     phip_resident_t *r = (phip_resident_t *)handle;
     if (!r) { phip_set_error("phip_resident_align: no batch"); return -1; }
+    if (!out) out = r->hout;
+    if (!out) { phip_set_error("phip_resident_align: no result buffer"); return -1; }
     pthread_mutex_lock(&g_lock);
     int rc = resident_run(r, alg, PHIP_SCORE, sg_flags, open, gap, 0);
     if (rc == 0) rc = resident_fetch(r, out, nullptr);

@@ -277,8 +277,7 @@ PHOST_EXPORT parasail_hip_batch_t *parasail_hip_batch_create(const char *s1, con
     }
     q = (uint8_t *)malloc((size_t)qtot);
     t = (uint8_t *)malloc((size_t)ttot);
-    hb->res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)npairs);
-    if (!q || !t || !hb->res) goto fail;
+    if (!q || !t) goto fail;
     for (int k = 0; k < npairs; ++k) {
         const char *a = s1 + s1_off[k], *b = s2 + s2_off[k];
         for (int i = 0; i < ql[k]; ++i) q[qo[k] + i] = (uint8_t)matrix->mapper[(unsigned char)a[i]];
@@ -324,24 +323,32 @@ PHOST_EXPORT int parasail_hip_batch_align(parasail_hip_batch_t *hb, const char *
         fprintf(stderr, "%s: invalid arguments\n", __func__);
         return -1;
     }
-    if (phip_resident_align(hb->dev, alg, flags, open, gap, hb->res) != 0) {
+    /* the results land in the device batch's pinned buffer when it has
+     * one, else in a buffer of our own */
+    const phip_out_t *res = phip_resident_results(hb->dev);
+    if (!res && !hb->res) {
+        hb->res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)hb->npairs);
+        if (!hb->res) return -1;
+    }
+    if (phip_resident_align(hb->dev, alg, flags, open, gap, res ? NULL : hb->res) != 0) {
         fprintf(stderr, "%s: %s\n", __func__, phip_last_error());
         return -1;
     }
+    if (!res) res = hb->res;
     if (!out) return 0;
+    int f0 = alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
+    if (alg == PHIP_SG) {
+        if (flags & PHIP_S1_BEG) f0 |= PARASAIL_FLAG_SG_S1_BEG;
+        if (flags & PHIP_S1_END) f0 |= PARASAIL_FLAG_SG_S1_END;
+        if (flags & PHIP_S2_BEG) f0 |= PARASAIL_FLAG_SG_S2_BEG;
+        if (flags & PHIP_S2_END) f0 |= PARASAIL_FLAG_SG_S2_END;
+    }
+    f0 |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
     for (int k = 0; k < hb->npairs; ++k) {
-        const phip_out_t *o = &hb->res[k];
+        const phip_out_t *o = &res[k];
         int w = (o->status & PHIP_ST_W32) ? 32 : 16;
         if (alg == PHIP_SW && w == 32 && !phost_saturates(alg, 16, open, hb->matrix, o)) w = 16;
-        int f = alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
-        if (alg == PHIP_SG) {
-            if (flags & PHIP_S1_BEG) f |= PARASAIL_FLAG_SG_S1_BEG;
-            if (flags & PHIP_S1_END) f |= PARASAIL_FLAG_SG_S1_END;
-            if (flags & PHIP_S2_BEG) f |= PARASAIL_FLAG_SG_S2_BEG;
-            if (flags & PHIP_S2_END) f |= PARASAIL_FLAG_SG_S2_END;
-        }
-        f |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
-        f |= w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32;
+        const int f = f0 | (w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32);
         if (out->score) out->score[k] = o->score;
         if (out->end_query) out->end_query[k] = o->end_query;
         if (out->end_ref) out->end_ref[k] = o->end_ref;

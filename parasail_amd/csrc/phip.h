@@ -98,7 +98,9 @@ int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, const uint8_t 
  * calls, and the wavefront plan for each scoring is built once.  Score-only
  * alignments; NULL / -1 on failure. */
 void *phip_resident_create(const phip_batch_t *b);
+/* out == NULL: into the handle's pinned buffer, phip_resident_results() */
 int phip_resident_align(void *handle, int alg, int sg_flags, int open, int gap, phip_out_t *out);
+const phip_out_t *phip_resident_results(void *handle);
 void phip_resident_free(void *handle);
 
 /* banded global alignment (src/nw_banded.c) for one pair */
