@@ -289,6 +289,12 @@ def long_pairs(peak32):
                                       "ops_per_cell": 7}},
             "nw_banded_k256": {"gcups": round(bcells / (kms_b / 1e3) / 1e9, 1), "cells": bcells,
                                "kernel_ms": round(kms_b, 1), "call_ms": round(wall_b * 1e3, 1),
+                               "kernel": "k_banded_rows (row bands of 64 R rows, 4 waves per pair)",
+                               "roofline": {"bound": "valu",
+                                            "achieved": round(bcells * 10 / (kms_b / 1e3) / 1e12, 2),
+                                            "peak": round(peak32 / 1e12, 2), "unit": "TOP/s",
+                                            "frac": round(bcells * 10 / (kms_b / 1e3) / peak32, 4),
+                                            "ops_per_cell": 10},
                                "score_checksum": int(rb["score"].astype(np.int64).sum())},
             "nw_score_checksum": int(r["score"].astype(np.int64).sum())}
 
