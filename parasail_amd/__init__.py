@@ -6,6 +6,7 @@ there is no Python or CPU fallback: when the shared library or the device is
 missing, calls raise.
 """
 import ctypes
+import time
 import os
 
 import numpy as np
@@ -203,15 +204,27 @@ def align(name, s1, s2, open_, gap, matrix):
 
 
 def cigar_tuple(c):
-    """(cigar string, beg_query, beg_ref) from a parasail_cigar_t*, freed."""
+    """(cigar string, beg_query, beg_ref) from a parasail_cigar_t*, freed;
+    the string is parasail_cigar_decode's."""
     if not c:
         return None
     cc = c.contents
-    ops = [cc.seq[k] for k in range(cc.len)]
-    s = "".join("%d%s" % (v >> 4, "MIDNSHP=XB"[v & 0xF]) for v in ops)
+    p = lib().parasail_cigar_decode(c)
+    s = ctypes.string_at(p).decode() if p else ""
+    if p:
+        lib()._libc_free(p)
     out = (s, cc.beg_query, cc.beg_ref)
     lib().parasail_cigar_free(c)
     return out
+
+
+_last_call_s = 0.0
+
+
+def last_call_ms():
+    """Wall time of the last align_arrays / align_batch library call (ms),
+    without the conversion of its results in Python."""
+    return _last_call_s * 1e3
 
 
 def align_arrays(alg, kind, width, batch, open_, gap, matrix, want_cigar=False):
@@ -234,10 +247,13 @@ def align_arrays(alg, kind, width, batch, open_, gap, matrix, want_cigar=False):
     toff = np.ascontiguousarray(batch.toff, dtype=np.int64)
     qlen = np.ascontiguousarray(batch.qlen, dtype=np.int32)
     tlen = np.ascontiguousarray(batch.tlen, dtype=np.int32)
+    global _last_call_s
+    t0 = time.perf_counter()
     rc = lib().parasail_hip_align_batch(
         alg.encode(), kind, width, batch.q, qoff.ctypes.data_as(I64), qlen.ctypes.data_as(I32),
         batch.t, toff.ctypes.data_as(I64), tlen.ctypes.data_as(I32), n, open_, gap, matrix,
         ctypes.byref(br))
+    _last_call_s = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError("parasail_hip_align_batch failed: %s"
                            % lib().parasail_hip_last_error().decode(errors="replace"))

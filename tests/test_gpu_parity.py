@@ -422,11 +422,12 @@ def test_banded_batch(mats):
 
 
 def test_banded_rows(mats):
-    """Pairs of >= 512 query rows take the row-band kernel: bands from 11 to
+    """Pairs of >= 512 query rows take the row-band kernel: bands from 3 to
     1,800 columns (every rows-per-lane choice), the query longer and shorter
-    than the target, bands reaching past both ends of the query, scores in
-    registers (DNA 2/-3, 5/2; a PSSM of it) and in an LDS table (BLOSUM62
-    11/1; DNA with a gap extension of 4 above an open of 2)."""
+    than the target, bands reaching past both ends of the query, targets of
+    1 to 40 columns, zero gap costs, scores in registers (DNA 2/-3, 5/2; a
+    PSSM of it) and in an LDS table (BLOSUM62 11/1; DNA with a gap extension
+    of 4 above an open of 2)."""
     L = P.lib()
     prng = random.Random(21)
     shapes = [(512, 530, 5), (700, 640, 40), (3000, 3100, 20), (2500, 2400, 200),
@@ -449,7 +450,16 @@ def test_banded_rows(mats):
                 exp = O.nw_banded(qs[i], ts[i], o, g, k, om)
                 assert (got["score"][x], got["end_query"][x], got["end_ref"][x]) == exp[:3], \
                     (key, o, g, len(qs[i]), len(ts[i]), k)
+    # targets of 1 to 40 columns, k = 1, free gaps, a 512-row query
     pm, om = mats["dna"]
+    edge = [(600, 1, 1, 5, 2), (700, 3, 2, 5, 2), (512, 40, 9, 5, 2), (900, 880, 1, 5, 2),
+            (1000, 1010, 30, 0, 0), (512, 512, 256, 3, 0)]
+    for (m, n, k, o, g) in edge:
+        s1 = rseq(prng, m, "ACGT")
+        s2 = rseq(prng, n, "ACGT")
+        got = P.nw_banded_batch([s1], [s2], o, g, k, pm)
+        exp = O.nw_banded(s1, s2, o, g, k, om)
+        assert (got["score"][0], got["end_query"][0], got["end_ref"][0]) == exp[:3], (m, n, k, o, g)
     for (m, n, k) in ((3000, 2950, 64), (800, 900, 300)):
         s1 = rseq(prng, m, "ACGT")
         s2 = mutate(prng, s1, "ACGT", sub=0.1, indel=0.02)[:n]
