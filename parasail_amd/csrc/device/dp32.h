@@ -52,3 +52,6 @@ struct CigarArgs {
     int32_t *nops, *beg;
 };
 int cigar_launch(const CigarArgs &a, hipStream_t st);
+// pair p's ops moved from opoff[p] to coff[p] of dst (after cigar_launch)
+int cigar_compact_launch(const uint32_t *ops, const int64_t *opoff, const int32_t *nops,
+                         const int64_t *coff, int np, uint32_t *dst, hipStream_t st);

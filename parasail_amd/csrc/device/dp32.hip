@@ -657,6 +657,30 @@ int dp32_launch(const DPParams &dp, const Dp32Plan &P, const char *region, hipSt
     return 0;
 }
 
+// Gather each pair's CIGAR ops (nops[p] of them at ops + opoff[p]; none if
+// nops[p] < 0) to dst + coff[p]: one wave per pair.
+__global__ void __launch_bounds__(256) k_cigar_compact(const uint32_t *ops, const int64_t *opoff,
+                                                       const int32_t *nops, const int64_t *coff,
+                                                       int np, uint32_t *dst)
+{
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (p >= np) return;
+    const int n = nops[p];
+    const uint32_t *src = ops + opoff[p];
+    uint32_t *out = dst + coff[p];
+    for (int i = lane; i < n; i += 64) out[i] = src[i];
+}
+
+int cigar_compact_launch(const uint32_t *ops, const int64_t *opoff, const int32_t *nops,
+                         const int64_t *coff, int np, uint32_t *dst, hipStream_t st)
+{
+    if (np <= 0) return 0;
+    hipLaunchKernelGGL(k_cigar_compact, dim3((np + 3) / 4), dim3(256), 0, st, ops, opoff, nops, coff,
+                       np, dst);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
 int cigar_launch(const CigarArgs &a, hipStream_t st)
 {
     if (a.npairs <= 0) return 0;

@@ -636,9 +636,28 @@ extern "C" int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, con
         if (resident_fetch(r, out, nullptr) != 0) break;
         ok = hipMemcpy(cig->nops, a.nops, (size_t)np * 4, hipMemcpyDeviceToHost) == hipSuccess;
         ok &= hipMemcpy(cig->beg, a.beg, (size_t)np * 8, hipMemcpyDeviceToHost) == hipSuccess;
-        ok &= hipMemcpy(cig->ops, a.ops, (size_t)otot * 4, hipMemcpyDeviceToHost) == hipSuccess;
         if (!ok) { phip_set_error("phip_align_cigars: download failed"); break; }
-        for (int i = 0; i < np; ++i) cig->off[i] = opoff[i];
+        // the ops reserve m + n + 2 slots a pair and use a few: gather the
+        // used ones back to back on the device and download only those
+        int64_t ctot = 0;
+        for (int i = 0; i < np; ++i) {
+            cig->off[i] = ctot;
+            ctot += cig->nops[i] > 0 ? cig->nops[i] : 0;
+        }
+        if (ctot > 0) {
+            int64_t *dco = (int64_t *)dev_alloc((size_t)np * 8);
+            uint32_t *dcomp = (uint32_t *)dev_alloc((size_t)ctot * 4);
+            ok = dco && dcomp;
+            ok = ok && hipMemcpyAsync(dco, cig->off, (size_t)np * 8, hipMemcpyHostToDevice, g_stream) ==
+                           hipSuccess;
+            ok = ok && cigar_compact_launch(a.ops, a.opoff, a.nops, dco, np, dcomp, g_stream) == 0;
+            ok = ok && hipMemcpyAsync(cig->ops, dcomp, (size_t)ctot * 4, hipMemcpyDeviceToHost,
+                                      g_stream) == hipSuccess;
+            ok = ok && hipStreamSynchronize(g_stream) == hipSuccess;
+            if (dco) dev_free(dco);
+            if (dcomp) dev_free(dcomp);
+            if (!ok) { phip_set_error("phip_align_cigars: download failed"); break; }
+        }
         rc = 0;
     } while (0);
     if (mem) dev_free(mem);

@@ -122,19 +122,15 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
                 free(q); free(t);
                 goto oom;
             }
-            for (int k = 0; k < npairs; ++k) {
-                const char *a = s1 + s1_off[k], *bb = s2 + s2_off[k];
-                for (int i = 0; i < ql[k]; ++i) qraw[qo[k] + i] = (uint8_t)toupper((unsigned char)a[i]);
-                for (int j = 0; j < tl[k]; ++j) traw[to[k] + j] = (uint8_t)toupper((unsigned char)bb[j]);
-            }
         }
-        for (int k = 0; k < npairs; ++k) {
+        {
+            /* codes, and the upper-cased residues for device CIGARs */
+            phost_enc_t e = {s2, s2_off, tl, to, matrix->mapper, t, traw};
+            phost_encode(&e, npairs);
             if (s1) {
-                const char *src = s1 + s1_off[k];
-                for (int i = 0; i < ql[k]; ++i) q[qo[k] + i] = (uint8_t)matrix->mapper[(unsigned char)src[i]];
+                phost_enc_t f = {s1, s1_off, ql, qo, matrix->mapper, q, qraw};
+                phost_encode(&f, npairs);
             }
-            const char *src = s2 + s2_off[k];
-            for (int j = 0; j < tl[k]; ++j) t[to[k] + j] = (uint8_t)matrix->mapper[(unsigned char)src[j]];
         }
         phip_batch_t b;
         memset(&b, 0, sizeof(b));
@@ -278,10 +274,11 @@ PHOST_EXPORT parasail_hip_batch_t *parasail_hip_batch_create(const char *s1, con
     q = (uint8_t *)malloc((size_t)qtot);
     t = (uint8_t *)malloc((size_t)ttot);
     if (!q || !t) goto fail;
-    for (int k = 0; k < npairs; ++k) {
-        const char *a = s1 + s1_off[k], *b = s2 + s2_off[k];
-        for (int i = 0; i < ql[k]; ++i) q[qo[k] + i] = (uint8_t)matrix->mapper[(unsigned char)a[i]];
-        for (int j = 0; j < tl[k]; ++j) t[to[k] + j] = (uint8_t)matrix->mapper[(unsigned char)b[j]];
+    {
+        phost_enc_t e = {s1, s1_off, ql, qo, matrix->mapper, q, NULL};
+        phost_encode(&e, npairs);
+        phost_enc_t f = {s2, s2_off, tl, to, matrix->mapper, t, NULL};
+        phost_encode(&f, npairs);
     }
     {
         phip_batch_t b;
@@ -413,11 +410,11 @@ PHOST_EXPORT int parasail_hip_nw_banded_batch(const char *s1, const int64_t *s1_
     q = (uint8_t *)malloc((size_t)qtot + 1);
     t = (uint8_t *)malloc((size_t)ttot + 1);
     if (!q || !t) goto done;
-    for (int p = 0; p < npairs; ++p) {
-        const unsigned char *a = (const unsigned char *)s1 + s1_off[p];
-        const unsigned char *b = (const unsigned char *)s2 + s2_off[p];
-        for (int i = 0; i < ql[p]; ++i) q[qo[p] + i] = (uint8_t)matrix->mapper[a[i]];
-        for (int j = 0; j < tl[p]; ++j) t[to[p] + j] = (uint8_t)matrix->mapper[b[j]];
+    {
+        phost_enc_t e = {s1, s1_off, ql, qo, matrix->mapper, q, NULL};
+        phost_encode(&e, npairs);
+        phost_enc_t f = {s2, s2_off, tl, to, matrix->mapper, t, NULL};
+        phost_encode(&f, npairs);
     }
     {
         phip_batch_t bt;

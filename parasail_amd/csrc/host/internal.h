@@ -26,6 +26,20 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
                                const parasail_profile_t *profile);
 
 /* saturation predicate of one width on exact DP values (align.c) */
+/* Codes (through mapper) and/or upper-cased copies of npairs sequences:
+ * pair k's len[k] residues at src + src_off[k] go to codes/upper +
+ * dst_off[k] (either destination may be NULL); host threads share the
+ * pairs when the batch is large. */
+typedef struct {
+    const char *src;
+    const int64_t *src_off;
+    const int32_t *len;
+    const int64_t *dst_off;
+    const int *mapper;
+    uint8_t *codes, *upper;
+} phost_enc_t;
+void phost_encode(const phost_enc_t *e, int npairs);
+
 int phost_saturates(int alg, int width, int open, const parasail_matrix_t *matrix,
                     const phip_out_t *o);
 
