@@ -344,9 +344,9 @@ __global__ void __launch_bounds__(64) k_banded_one(BandCfg c, int *scratch, int6
 // step count) after band b-1 reached the same column, and the waves meet at
 // a barrier every RB_K steps.  Wave w's lane 63 stores its bottom row at step
 // s in slot s of the wave's LDS row (two steps per store); lane 0 of the
-// next band loads, two steps ahead and two steps per load, what was stored
-// at least G - 64 R - 68 >= RB_K steps earlier, so every value it loads was
-// stored before the last barrier.  A wave's next band starts NWV G steps
+// next band loads, two steps ahead, what was stored at least
+// G - 64 R - 68 >= RB_K steps earlier, so every value it loads was stored
+// before the last barrier.  A wave's next band starts NWV G steps
 // after its last one, at least a band's steps plus RB_K: each wave holds
 // one band per stretch between barriers, and overwrites a slot long after
 // the next band loaded it.
@@ -465,30 +465,33 @@ __device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, cons
             botF = v ? F : RNEG;
         }
     };
-    auto run = [&](auto slowc, int s, const int e) __attribute__((always_inline)) {
+    // lane 0's row above at step s + 2, loaded into the register of step s
+    // once step s has used it
+    auto ld_next = [&](auto slowc, const int s) -> int2 {
         constexpr bool SLOW = decltype(slowc)::value;
+        return SLOW && b == 0 ? top(s + 2) : ring_in[s + 2];
+    };
+    auto run = [&](auto slowc, int s, int e) __attribute__((always_inline)) {
+        // the step counters are the wave's own (scalar registers)
+        s = __builtin_amdgcn_readfirstlane(s);
+        e = __builtin_amdgcn_readfirstlane(e);
         while (s < e) {
             if ((s & 63) == 0 && s != 0) {
                 cbuf = RP ? 4 * cnext : cnext;
                 cnext = ldc(a + s + 64 + t);
             }
             const int ce = min(e, (s | 63) + 1);
-        for (; s < ce; s += 2) {
-            int2 nx0, nx1;
-            if (SLOW && b == 0) {
-                nx0 = top(s + 2);
-                nx1 = top(s + 3);
-            } else {
-                nx0 = ring_in[s + 2];
-                nx1 = ring_in[s + 3];
+            for (; s < ce; s += 2) {
+                step(slowc, s, in0);
+                const int h0 = botH, f0 = botF;
+                in0 = ld_next(slowc, s);
+                // keep each load a whole step ahead of its use (the
+                // scheduler would otherwise pair the two loads late)
+                __builtin_amdgcn_sched_barrier(0);
+                step(slowc, s + 1, in1);
+                in1 = ld_next(slowc, s + 1);
+                if (t == 63) *(int4 *)(ring_out + s) = make_int4(h0, f0, botH, botF);
             }
-            step(slowc, s, in0);
-            const int h0 = botH, f0 = botF;
-            step(slowc, s + 1, in1);
-            if (t == 63) *(int4 *)(ring_out + s) = make_int4(h0, f0, botH, botF);
-            in0 = nx0;
-            in1 = nx1;
-        }
         }
     };
     const std::integral_constant<bool, false> fastc;
@@ -710,7 +713,7 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         const int64_t low = (m > n) ? (int64_t)n - m - k : (int64_t)m - n - k;
         const int64_t colLen = k - low + 1;
         maxcol = std::max(maxcol, colLen);
-        if (rows_ok && m >= 512 && colLen <= 1 << 14 &&
+        if (rows_ok && m >= 512 && m <= 1 << 24 && colLen <= 1 << 14 &&
             ((int64_t)m + n + 2) * ((int64_t)b->open + b->gap + wabs) < (1 << 27)) {
             const int R = rows_pick(colLen, m);
             const int64_t steps = R ? rows_band_steps(R, colLen) : 0;
