@@ -53,8 +53,6 @@ constexpr int kGroupRows = LW * 64 * kLongR;
 constexpr int kSpinLimit = 1 << 23;
 constexpr int kPubPhases = 4;  // barriers between progress publications (power of two)
 
-__device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, false); }
-
 }  // namespace
 
 struct LongItem {
@@ -97,7 +95,9 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
     for (int x = threadIdx.x; x < tabn; x += blockDim.x)
         lds[x] = x < msize * msize ? c.mat[x] + open : LWNEG;
     int2 *const ring = (int2 *)(lds + ((tabn + 1) & ~1));
-    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    // the wave's band (a scalar: every band-level branch and index below is
+    // the wave's own)
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
     const int2 *ring_in = ring + (w > 0 ? w - 1 : 0) * LRING;
     int2 *ring_out = ring + (w < LW - 1 ? w : 0) * LRING;
     int *const failed = c.ctl + 1;
@@ -168,27 +168,31 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
         }
         int prevRH = left(first_row - 1);
         int botH = LNEG, botF = LNEG;
-        // lane 0's input for column jj: the DP's row above the query, the
-        // previous group's last band (HBM) or this group's band above (ring)
-        auto ld_in = [&](int jj) -> int2 {
+        // lane 0's input for column jj: this group's band above (MODE 0, the
+        // LDS ring), the previous group's last band (1, HBM) or the DP's row
+        // above the query (2); a template, so that the loads of a step never
+        // meet at a branch
+        auto ld_in = [&](auto modec, int jj) -> int2 {
+            constexpr int MODE = decltype(modec)::value;
             const int jc = jj < 0 ? 0 : (jj >= n ? n - 1 : jj);
-            if (w == 0) {
-                if (!gin) return make_int2(-open - jc * gap - open, LNEG);
-                return gin[jc];
-            }
+            if (MODE == 2) return make_int2(-open - jc * gap - open, LNEG);
+            if (MODE == 1) return gin[jc];
             return ring_in[jc & (LRING - 1)];
         };
         auto ld_t = [&](int jj) -> int { return (int)s2[jj < 0 ? 0 : (jj >= n ? n - 1 : jj)]; };
         const int s0 = -LL * w;  // this band's step at group step 0
-        wait_rows(0);
-        int2 in0 = ld_in(s0), in1 = ld_in(s0 + 1);
-        int tc0 = ld_t(s0 - t), tc1 = ld_t(s0 + 1 - t);
         const bool pub = gout && w == LW - 1 && t == 63;
 
-        auto step = [&](const int s, int2 &bx, int &tq) __attribute__((always_inline)) {
+        // FAST: every lane's column lies in [0, n) (the band's steady state)
+        auto step = [&](auto modec, auto fastc, const int s, int2 &bx, int &tq)
+                        __attribute__((always_inline)) {
+            constexpr bool FAST = decltype(fastc)::value;
             const int j = s - t;
-            const int xH = shr1(botH), xF = shr1(botF);
-            if (j >= 0 && j < n) {
+            // lane 0 takes the row above from bx, every other lane from its
+            // left neighbour
+            const int rH = __builtin_amdgcn_update_dpp(bx.x, botH, 0x138, 0xF, 0xF, false);
+            const int rF = __builtin_amdgcn_update_dpp(bx.y, botF, 0x138, 0xF, 0xF, false);
+            if (FAST || (j >= 0 && j < n)) {
                 const int code = tq;
                 int W[R];
                 if (RP) {
@@ -199,8 +203,6 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
 #pragma unroll
                     for (int i = 0; i < R; ++i) W[i] = lds[qo[i] + code];
                 }
-                const int rH = t == 0 ? bx.x : xH;
-                const int rF = t == 0 ? bx.y : xF;
                 int t1 = prevRH + W[0];
                 prevRH = rH;
                 int up = rH, F = rF;
@@ -221,27 +223,47 @@ __global__ void __launch_bounds__(LW * 64) k_nwlong(LongCfg c)
                 if (t == 63 && w < LW - 1) ring_out[j & (LRING - 1)] = make_int2(botH, botF);
                 if (pub) gout[j] = make_int2(botH, botF);
             }
-            bx = ld_in(s + 2);
+            bx = ld_in(modec, s + 2);
             tq = ld_t(j + 2);
         };
-        for (int ph = 0; ph < nph; ++ph) {
-            if (live) {
-                const int sb = ph * LK + s0;
-                for (int k = 0; k < LK; k += 2) {
-                    step(sb + k, in0, tc0);
-                    step(sb + k + 1, in1, tc1);
+        const std::integral_constant<bool, true> fastc;
+        const std::integral_constant<bool, false> slowc;
+        auto run = [&](auto modec) {
+            wait_rows(0);
+            int2 in0 = ld_in(modec, s0), in1 = ld_in(modec, s0 + 1);
+            int tc0 = ld_t(s0 - t), tc1 = ld_t(s0 + 1 - t);
+            for (int ph = 0; ph < nph; ++ph) {
+                if (live) {
+                    const int sb = ph * LK + s0;
+                    if (sb >= 63 && sb + LK <= n) {
+                        for (int k = 0; k < LK; k += 2) {
+                            step(modec, fastc, sb + k, in0, tc0);
+                            step(modec, fastc, sb + k + 1, in1, tc1);
+                        }
+                    } else {
+                        for (int k = 0; k < LK; k += 2) {
+                            step(modec, slowc, sb + k, in0, tc0);
+                            step(modec, slowc, sb + k + 1, in1, tc1);
+                        }
+                    }
+                    if (pub && ((ph & (kPubPhases - 1)) == kPubPhases - 1 || ph + 1 == nph)) {
+                        // columns of the bottom row stored so far, then
+                        // published (every kPubPhases barriers: each release
+                        // writes back this CU's L2)
+                        const int done = min(n, max(0, (ph + 1) * LK - LL * (LW - 1) - 63));
+                        __hip_atomic_store(c.prog + item, done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    }
                 }
-                if (pub && ((ph & (kPubPhases - 1)) == kPubPhases - 1 || ph + 1 == nph)) {
-                    // columns of the bottom row stored so far, then published
-                    // (every kPubPhases barriers: each release writes back
-                    // this CU's L2)
-                    const int done = min(n, max(0, (ph + 1) * LK - LL * (LW - 1) - 63));
-                    __hip_atomic_store(c.prog + item, done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                }
+                if (ph + 1 < nph) wait_rows(ph + 1);
+                lds_barrier();
             }
-            if (ph + 1 < nph) wait_rows(ph + 1);
-            __syncthreads();
-        }
+        };
+        if (w > 0)
+            run(std::integral_constant<int, 0>());
+        else if (gin)
+            run(std::integral_constant<int, 1>());
+        else
+            run(std::integral_constant<int, 2>());
         // a lane stops updating after its last column, so its rows now hold
         // column n-1: the score is H[m-1][n-1] (nw.c:155-157)
         if (b == bm && t == tm) {
