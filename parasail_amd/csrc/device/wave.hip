@@ -219,11 +219,13 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         // fetched two steps earlier into the same registers, and the next
         // ones are fetched after their last use, so no value is copied and
         // each wait is for a load issued more than a step before
-        auto step = [&](const int sv, uint32_t &cx, int2 &bx) __attribute__((always_inline)) {
+        // FAST: every lane's column lies in [0, width) (the steady state)
+        auto step = [&](auto fastc, const int sv, uint32_t &cx, int2 &bx) __attribute__((always_inline)) {
+            constexpr bool FAST = decltype(fastc)::value;
             const int j = sv - t;
             const int xH = shr1(botH);  // every lane enabled
             const int xF = shr1(botF);
-            if (j >= 0 && j < width) {
+            if (FAST || (j >= 0 && j < width)) {
                 const int toA = cx & 0xffff, toB = cx >> 16;
                 int W[R];
 #pragma unroll
@@ -309,9 +311,23 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         };
         uint32_t c0 = ccl[0], c1 = ccl[1];
         int2 b0 = bbs[0], b1 = bbs[1];
-        for (int s = 0; s < steps; s += 2) {
-            step(s, c0, b0);
-            step(s + 1, c1, b1);
+        const std::integral_constant<bool, true> fastc;
+        const std::integral_constant<bool, false> slowc;
+        // pairs of steps whose every lane is inside the band's columns: the
+        // even s with s >= T - 1 and s + 1 < width
+        const int lo = min(T, steps), hi = max(lo, width - 1);
+        int s = 0;
+        for (; s < lo; s += 2) {
+            step(slowc, s, c0, b0);
+            step(slowc, s + 1, c1, b1);
+        }
+        for (; s < hi; s += 2) {
+            step(fastc, s, c0, b0);
+            step(fastc, s + 1, c1, b1);
+        }
+        for (; s < steps; s += 2) {
+            step(slowc, s, c0, b0);
+            step(slowc, s + 1, c1, b1);
         }
         if (SW && band_live) {
 #pragma unroll
