@@ -111,11 +111,13 @@ __device__ __forceinline__ void search_wave(const SearchCfg &c, const DbWave &wd
         int2 *const bbr = c.bandbuf + J.bboff;
         const short *prow = prof + r0;
 
-        auto step = [&](const int sv, uint32_t &cx, int2 &bx) __attribute__((always_inline)) {
+        // FAST: every lane's column lies in [0, width) (the steady state)
+        auto step = [&](auto fastc, const int sv, uint32_t &cx, int2 &bx) __attribute__((always_inline)) {
+            constexpr bool FAST = decltype(fastc)::value;
             const int j = sv - t;
             const int xH = pk::shr1(botH);
             const int xF = pk::shr1(botF);
-            if (j >= 0 && j < width) {
+            if (FAST || (j >= 0 && j < width)) {
                 const uint2 *pa = (const uint2 *)(prow + (cx & 0xff) * mp);
                 const uint2 *pb = (const uint2 *)(prow + (cx >> 8) * mp);
                 int W[R];
@@ -169,9 +171,23 @@ __device__ __forceinline__ void search_wave(const SearchCfg &c, const DbWave &wd
         };
         uint32_t c0 = ccl[0], c1 = ccl[1];
         int2 b0 = bbs[0], b1 = bbs[1];
-        for (int s = 0; s < steps; s += 2) {
-            step(s, c0, b0);
-            step(s + 1, c1, b1);
+        const std::integral_constant<bool, true> fastc;
+        const std::integral_constant<bool, false> slowc;
+        // pairs of steps whose every lane is inside the columns: the even s
+        // with s >= T - 1 and s + 1 < width
+        const int lo = min(T, steps), hi = max(lo, width - 1);
+        int s = 0;
+        for (; s < lo; s += 2) {
+            step(slowc, s, c0, b0);
+            step(slowc, s + 1, c1, b1);
+        }
+        for (; s < hi; s += 2) {
+            step(fastc, s, c0, b0);
+            step(fastc, s + 1, c1, b1);
+        }
+        for (; s < steps; s += 2) {
+            step(slowc, s, c0, b0);
+            step(slowc, s + 1, c1, b1);
         }
         if (live) {
 #pragma unroll
