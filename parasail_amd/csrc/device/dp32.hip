@@ -122,11 +122,13 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
     auto ld_t = [&](int j) -> int { return (int)s2[j < 0 ? 0 : (j >= nreal ? nreal - 1 : j)]; };
     int tq0 = ld_t(-t), tq1 = ld_t(1 - t);
 
-    auto step = [&](const int s, int &tq) __attribute__((always_inline)) {
+    // FAST: every lane's column lies in [0, n) of its job (the steady state)
+    auto step = [&](auto fastc, const int s, int &tq) __attribute__((always_inline)) {
+        constexpr bool FAST = decltype(fastc)::value;
         const int j = s - t;
         const int xH = shr1(botH), xF = shr1(botF);
         const int xS = STATS ? shr1(botS) : 0, xFS = STATS ? shr1(botFS) : 0;
-        if (j >= 0 && j < n) {
+        if (FAST || (j >= 0 && j < n)) {
             const int tc = tq;
             int2 Wv[R];
 #pragma unroll
@@ -233,9 +235,27 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
         tq = ld_t(j + 2);
     };
     const int steps = wd.steps;
-    for (int s = 0; s < steps; s += 2) {
-        step(s, tq0);
-        step(s + 1, tq1);
+    // the wave's shortest job (its lane-0 values; an empty slot has n = 0)
+    int nmin = n;
+    for (int k = T; k < 64; k += T) nmin = min(nmin, __builtin_amdgcn_readlane(n, k));
+    nmin = min(__builtin_amdgcn_readfirstlane(n), __builtin_amdgcn_readfirstlane(nmin));
+    const std::integral_constant<bool, true> fastc;
+    const std::integral_constant<bool, false> slowc;
+    // pairs of steps whose every lane is inside its job's columns: the even
+    // s with s >= T - 1 and s + 1 < nmin
+    const int lo = min(T, steps), hi = max(lo, nmin - 1);
+    int s = 0;
+    for (; s < lo; s += 2) {
+        step(slowc, s, tq0);
+        step(slowc, s + 1, tq1);
+    }
+    for (; s < hi; s += 2) {
+        step(fastc, s, tq0);
+        step(fastc, s + 1, tq1);
+    }
+    for (; s < steps; s += 2) {
+        step(slowc, s, tq0);
+        step(slowc, s + 1, tq1);
     }
 
     // reduce over the job's lanes, then lane 0 applies the reference's end
