@@ -250,7 +250,8 @@ __global__ void __launch_bounds__(512) k_banded_chain(BandCfg c)
     int2 *const ux = lx + 8;
     __syncthreads();
     if (blockIdx.x >= (unsigned)c.npairs) return;  // whole block
-    const int NWV = blockDim.x >> 6, w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int NWV = blockDim.x >> 6, t = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int pr = c.pairs[blockIdx.x];
 #define BAND_R(RR) band_chain<(RMAX >= RR ? RR : 2), LTAB>(c, tab, pr, w, t, NWV, lx, ux)
     switch (c.rsel[blockIdx.x]) {

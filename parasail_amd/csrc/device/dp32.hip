@@ -363,7 +363,8 @@ __global__ void __launch_bounds__(256) k_dp32(Dp32Cfg c)
         }
     }
     __syncthreads();
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave's queue entry (a scalar: its loop bounds are the wave's own)
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (w >= c.nwaves) return;  // whole wave
     const Dp32Wave wd = c.waves[w];
     const int lane = threadIdx.x & 63;
