@@ -155,9 +155,14 @@ struct phip_resident {
     std::vector<int32_t> slow, tlay;
     int32_t *dslow = nullptr, *dtlay = nullptr;
     phip_out_t *hout = nullptr;  // pinned host copy of the results (NULL: none)
+    // score-only results packed to 16 bytes a pair (device, then pinned host)
+    phip_out4_t *dout4 = nullptr, *hout4 = nullptr;
+    bool hout4_pinned = false;
 };
 
 typedef struct phip_resident phip_resident_t;
+
+static void resident_free(phip_resident_t *r);
 
 static phip_resident_t *resident_create(const phip_batch_t *b)
 {
@@ -193,7 +198,8 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
     size_t o_mat = slot(r->mvals.size() * 4), o_q = slot(r->has_q ? qtot : 0), o_t = slot(ttot),
            o_qo = slot(np * 8), o_to = slot(np * 8), o_ql = slot(np * 4), o_tl = slot(np * 4),
            o_out = slot(np * sizeof(phip_out_t)), o_tro = slot(r->troff.size() * 8),
-           o_tr = slot(r->trtot), o_slow = slot(np * 4), o_tlay = slot(np * 4);
+           o_tr = slot(r->trtot), o_slow = slot(np * 4), o_tlay = slot(np * 4),
+           o_out4 = slot(np * sizeof(phip_out4_t));
     r->base = (char *)dev_alloc(off);
     if (!r->base) { delete r; return nullptr; }
     r->dmat = (int *)(r->base + o_mat);
@@ -208,6 +214,7 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
     r->dtr = b->kind == PHIP_TRACE ? (int8_t *)(r->base + o_tr) : nullptr;
     r->dslow = (int32_t *)(r->base + o_slow);
     r->dtlay = (int32_t *)(r->base + o_tlay);
+    r->dout4 = (phip_out4_t *)(r->base + o_out4);
     bool ok = true;
     ok &= hipMemcpyAsync(r->dmat, r->mvals.data(), r->mvals.size() * 4, hipMemcpyHostToDevice, g_stream) == hipSuccess;
     if (r->has_q) ok &= hipMemcpyAsync(r->dq, b->q, qtot, hipMemcpyHostToDevice, g_stream) == hipSuccess;
@@ -232,6 +239,17 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
         (void)hipGetLastError();
         r->hout = nullptr;
     }
+    r->hout4_pinned = hipHostMalloc((void **)&r->hout4, (size_t)np * sizeof(phip_out4_t),
+                                    hipHostMallocDefault) == hipSuccess;
+    if (!r->hout4_pinned) {
+        (void)hipGetLastError();
+        r->hout4 = (phip_out4_t *)malloc((size_t)np * sizeof(phip_out4_t));
+    }
+    if (!r->hout4) {
+        phip_set_error("resident batch: out of host memory");
+        resident_free(r);
+        return nullptr;
+    }
     return r;
 }
 
@@ -242,6 +260,12 @@ static void resident_free(phip_resident_t *r)
     if (r->dplan_mem) dev_free(r->dplan_mem);
     dev_free(r->base);
     if (r->hout) (void)hipHostFree(r->hout);
+    if (r->hout4) {
+        if (r->hout4_pinned)
+            (void)hipHostFree(r->hout4);
+        else
+            free(r->hout4);
+    }
     delete r;
 }
 
@@ -674,6 +698,51 @@ extern "C" void *phip_resident_create(const phip_batch_t *b)
     phip_resident_t *r = resident_create(b);
     pthread_mutex_unlock(&g_lock);
     return r;
+}
+
+__global__ void k_pack_out4(const phip_out_t *out, phip_out4_t *o4, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const phip_out_t o = out[i];
+    o4[i] = phip_out4_t{o.score, o.end_query, o.end_ref, o.status};
+}
+
+extern "C" int phip_resident_align4(void *handle, int alg, int sg_flags, int open, int gap,
+                                    const phip_out4_t **res)
+{
+    phip_resident_t *r = (phip_resident_t *)handle;
+    if (!r || !res) { phip_set_error("phip_resident_align4: bad arguments"); return -1; }
+    pthread_mutex_lock(&g_lock);
+    int rc = resident_run(r, alg, PHIP_SCORE, sg_flags, open, gap, 0);
+    do {
+        if (rc != 0) break;
+        rc = -1;
+        hipLaunchKernelGGL(k_pack_out4, dim3((r->np + 255) / 256), dim3(256), 0, g_stream, r->dout,
+                           r->dout4, r->np);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(r->hout4, r->dout4, (size_t)r->np * sizeof(phip_out4_t),
+                           hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+            hipStreamSynchronize(g_stream) != hipSuccess) {
+            phip_set_error("phip_resident_align4: download failed");
+            break;
+        }
+        bool bad = false;
+        for (int i = 0; i < r->np && !bad; ++i)
+            if (r->hout4[i].status & PHIP_ST_ERR) {
+                phip_set_error("pair %d could not be computed (long-pair pipeline failed)", i);
+                bad = true;
+            }
+        if (bad) break;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, g_ev0, g_ev1) != hipSuccess) break;
+        g_last_ms = ms;
+        g_last_main_ms = r->last_wave ? wave_last_kernel_ms() : ms;
+        *res = r->hout4;
+        rc = 0;
+    } while (0);
+    pthread_mutex_unlock(&g_lock);
+    return rc;
 }
 
 extern "C" const phip_out_t *phip_resident_results(void *handle)

@@ -230,7 +230,6 @@ struct parasail_hip_batch {
     int npairs;
     phip_matrix_t mx;
     const parasail_matrix_t *matrix;
-    phip_out_t *res;
 };
 
 PHOST_EXPORT parasail_hip_batch_t *parasail_hip_batch_create(const char *s1, const int64_t *s1_off,
@@ -307,7 +306,6 @@ PHOST_EXPORT parasail_hip_batch_t *parasail_hip_batch_create(const char *s1, con
     free(q); free(t); free(ql); free(tl); free(qo); free(to);
     return hb;
 fail:
-    if (hb) free(hb->res);
     free(hb); free(q); free(t); free(ql); free(tl); free(qo); free(to);
     return NULL;
 }
@@ -320,18 +318,13 @@ PHOST_EXPORT int parasail_hip_batch_align(parasail_hip_batch_t *hb, const char *
         fprintf(stderr, "%s: invalid arguments\n", __func__);
         return -1;
     }
-    /* the results land in the device batch's pinned buffer when it has
-     * one, else in a buffer of our own */
-    const phip_out_t *res = phip_resident_results(hb->dev);
-    if (!res && !hb->res) {
-        hb->res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)hb->npairs);
-        if (!hb->res) return -1;
-    }
-    if (phip_resident_align(hb->dev, alg, flags, open, gap, res ? NULL : hb->res) != 0) {
+    /* score, ends and status of each pair, 16 bytes a pair, in the device
+     * batch's pinned buffer */
+    const phip_out4_t *res = NULL;
+    if (phip_resident_align4(hb->dev, alg, flags, open, gap, &res) != 0) {
         fprintf(stderr, "%s: %s\n", __func__, phip_last_error());
         return -1;
     }
-    if (!res) res = hb->res;
     if (!out) return 0;
     int f0 = alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
     if (alg == PHIP_SG) {
@@ -342,9 +335,14 @@ PHOST_EXPORT int parasail_hip_batch_align(parasail_hip_batch_t *hb, const char *
     }
     f0 |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
     for (int k = 0; k < hb->npairs; ++k) {
-        const phip_out_t *o = &res[k];
+        const phip_out4_t *o = &res[k];
         int w = (o->status & PHIP_ST_W32) ? 32 : 16;
-        if (alg == PHIP_SW && w == 32 && !phost_saturates(alg, 16, open, hb->matrix, o)) w = 16;
+        if (alg == PHIP_SW && w == 32) {
+            phip_out_t full;
+            memset(&full, 0, sizeof(full));
+            full.score = o->score;  /* the SW width test reads the score only */
+            if (!phost_saturates(alg, 16, open, hb->matrix, &full)) w = 16;
+        }
         const int f = f0 | (w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32);
         if (out->score) out->score[k] = o->score;
         if (out->end_query) out->end_query[k] = o->end_query;
@@ -358,7 +356,6 @@ PHOST_EXPORT void parasail_hip_batch_free(parasail_hip_batch_t *hb)
 {
     if (!hb) return;
     phip_resident_free(hb->dev);
-    free(hb->res);
     free(hb);
 }
 

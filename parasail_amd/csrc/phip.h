@@ -98,6 +98,14 @@ int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, const uint8_t 
  * calls, and the wavefront plan for each scoring is built once.  Score-only
  * alignments; NULL / -1 on failure. */
 void *phip_resident_create(const phip_batch_t *b);
+/* Score-only results of a resident batch, 16 bytes a pair. */
+typedef struct phip_out4 {
+    int32_t score, end_query, end_ref, status;
+} phip_out4_t;
+/* Score-only pass with the results packed on the device and downloaded into
+ * the handle's pinned buffer (*res, valid until the next call or free). */
+int phip_resident_align4(void *handle, int alg, int sg_flags, int open, int gap,
+                         const phip_out4_t **res);
 /* out == NULL: into the handle's pinned buffer, phip_resident_results() */
 int phip_resident_align(void *handle, int alg, int sg_flags, int open, int gap, phip_out_t *out);
 const phip_out_t *phip_resident_results(void *handle);
