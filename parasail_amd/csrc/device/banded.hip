@@ -357,10 +357,11 @@ constexpr int RNEG = -(1 << 29);  // -inf: the host bounds every score above it
 static_assert(RB_L >= RB_K + 67, "band lag too short for the barrier spacing");
 
 // steps between consecutive row bands of 64 R = br rows taking send steps
-// each, on nwv waves (even, like every band start)
+// each, on nwv waves (a multiple of 4, like every band's steps: the step
+// loop runs four steps a turn)
 __host__ __device__ __forceinline__ int rows_spacing(int br, int send, int nwv)
 {
-    const int q = ((send + RB_K + nwv - 1) / nwv + 1) & ~1;
+    const int q = ((send + RB_K + nwv - 1) / nwv + 3) & ~3;
     return br + RB_L > q ? br + RB_L : q;
 }
 
@@ -388,7 +389,8 @@ __device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, cons
     const int nb = (m + BR - 1) / BR;
     // a band's steps: lane 63 passes its last band column, then stores
     // -inf for the R + 2 columns lane 0 of the next band still loads
-    const int send = (BR + colLen + R + 64 + 1) & ~1;
+    // (rounded up to a multiple of 4)
+    const int send = (BR + colLen + R + 64 + 3) & ~3;
     int2 *const ring_out = rings + w * c.lring;
     // slot of the column lane 0 computes at step s: BR + s + 63
     const int2 *const ring_in = rings + (w == 0 ? NWV - 1 : w - 1) * c.lring + BR + 63;
@@ -400,6 +402,9 @@ __device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, cons
     const int tp = t * (R + 1);
 
     int HL[R], E[R], qp[R];  // H - open and E of the lane's rows; their scores
+    // fast bands: whether row 0 was in the band 1, 2, ... steps ago (row i
+    // is in the band at step s iff row 0 was at step s - i)
+    bool vh[R > 1 ? R - 1 : 1];
     int prevRH = RNEG, botH = RNEG, botF = RNEG, tq = 0, leftTop = RNEG, r0 = 0, a = 0;
     // cbuf: codes of the 64 columns from lane 0's, rotated one lane a step
     // (lane 0 holds lane 0's column); cnext: the next 64, as loaded
@@ -450,10 +455,13 @@ __device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, cons
         if (act) {
             int up = xH, F = xF;
             bool v = false;
+            const bool v0 = (unsigned)p < (unsigned)colLen;
 #pragma unroll
             for (int i = 0; i < R; ++i) {
-                v = (unsigned)(p - i) < (unsigned)colLen;
-                if (SLOW) v = v && r0 + i < m;
+                if (SLOW)
+                    v = (unsigned)(p - i) < (unsigned)colLen && r0 + i < m;
+                else
+                    v = i == 0 ? v0 : vh[i - 1];  // row i now = row 0 i steps ago
                 E[i] = max(E[i] - gap, HL[i]);
                 const int t1n = i + 1 < R ? HL[i] + W[i + 1] : 0;
                 F = max(F - gap, up);
@@ -464,6 +472,11 @@ __device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, cons
             }
             botH = HL[R - 1];
             botF = v ? F : RNEG;
+            if (!SLOW && R > 1) {
+#pragma unroll
+                for (int k = R - 2; k > 0; --k) vh[k] = vh[k - 1];
+                vh[0] = v0;
+            }
         }
     };
     // lane 0's row above at step s + 2, loaded into the register of step s
@@ -482,16 +495,19 @@ __device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, cons
                 cnext = ldc(a + s + 64 + t);
             }
             const int ce = min(e, (s | 63) + 1);
-            for (; s < ce; s += 2) {
-                step(slowc, s, in0);
-                const int h0 = botH, f0 = botF;
-                in0 = ld_next(slowc, s);
-                // keep each load a whole step ahead of its use (the
-                // scheduler would otherwise pair the two loads late)
-                __builtin_amdgcn_sched_barrier(0);
-                step(slowc, s + 1, in1);
-                in1 = ld_next(slowc, s + 1);
-                if (t == 63) *(int4 *)(ring_out + s) = make_int4(h0, f0, botH, botF);
+            for (; s < ce; s += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; u += 2) {
+                    step(slowc, s + u, in0);
+                    const int h0 = botH, f0 = botF;
+                    in0 = ld_next(slowc, s + u);
+                    // keep each load a whole step ahead of its use (the
+                    // scheduler would otherwise pair the two loads late)
+                    __builtin_amdgcn_sched_barrier(0);
+                    step(slowc, s + u + 1, in1);
+                    in1 = ld_next(slowc, s + u + 1);
+                    if (t == 63) *(int4 *)(ring_out + s + u) = make_int4(h0, f0, botH, botF);
+                }
             }
         }
     };
@@ -525,6 +541,8 @@ __device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, cons
                         HL[i] = r < m && r <= lb ? -open - r * gap - open : RNEG;
                         E[i] = RNEG;
                     }
+#pragma unroll
+                    for (int k = 0; k < (R > 1 ? R - 1 : 1); ++k) vh[k] = false;  // steps < 0
                     const int rt = r0 - 1;
                     leftTop = rt == -1 ? -open : (rt <= lb ? -open - rt * gap - open : RNEG);
                     botH = botF = RNEG;
@@ -630,7 +648,7 @@ const int kRowsRs[] = {1, 2, 3, 4, 6, 8};
 constexpr size_t kLdsMax = 160 * 1024;
 
 // steps of one row band (band_rows' send)
-int64_t rows_band_steps(int R, int64_t colLen) { return (64 * R + colLen + R + 64 + 1) & ~(int64_t)1; }
+int64_t rows_band_steps(int R, int64_t colLen) { return (64 * R + colLen + R + 64 + 3) & ~(int64_t)3; }
 // bottom-row slots a wave's LDS row needs: the band's steps, plus the
 // BR + 66 further slots lane 0 of the next band loads past them
 int64_t rows_slots(int R, int64_t colLen) { return rows_band_steps(R, colLen) + 64 * R + 66; }
