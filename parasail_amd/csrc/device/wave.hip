@@ -418,27 +418,39 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // SW epilogue: reduce T lane candidates per pair (max score, then min column,
 // then min row: src/sw.c:150-158).  ovf_limit: the int16 tier's largest safe
 // score; pairs above it are appended to `redo` for the int32 tier.
-__global__ void k_sw_reduce(const int4 *cand, const PairRef *refs, int n, int ovf_limit,
-                            phip_out_t *out, int32_t *redo, int *nredo)
+__global__ void __launch_bounds__(256) k_sw_reduce(const int4 *cand, const PairRef *refs, int n,
+                                                   int ovf_limit, phip_out_t *out, int32_t *redo,
+                                                   int *nredo)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const PairRef r = refs[k];
-    const int4 *c = cand + r.cdoff;
+    // 16 lanes per pair; the order is total, so the lanes' partial picks
+    // combine to the serial pick in any order
+    const int k = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int l = threadIdx.x & 15;
     int S = -1, col = 0, row = 0;
-    for (int t = 0; t < r.T; ++t) {
-        int4 v = c[t];
-        if (v.x > S || (v.x == S && (v.y < col || (v.y == col && v.z < row)))) {
-            S = v.x; col = v.y; row = v.z;
+    if (k < n) {
+        const PairRef r = refs[k];
+        for (int t = l; t < r.T; t += 16) {
+            const int4 v = cand[r.cdoff + t];
+            if (v.x > S || (v.x == S && (v.y < col || (v.y == col && v.z < row)))) {
+                S = v.x; col = v.y; row = v.z;
+            }
         }
     }
+    for (int d = 8; d >= 1; d >>= 1) {
+        const int oS = __shfl_xor(S, d), oc = __shfl_xor(col, d), orow = __shfl_xor(row, d);
+        if (oS > S || (oS == S && (oc < col || (oc == col && orow < row)))) {
+            S = oS; col = oc; row = orow;
+        }
+    }
+    if (k >= n || l) return;
+    const int pair = refs[k].pair;
     phip_out_t o;
     memset(&o, 0, sizeof(o));
     o.score = S;
     o.end_query = row;
     o.end_ref = col;
-    out[r.pair] = o;
-    if (S > ovf_limit) redo[atomicAdd(nredo, 1)] = r.pair;
+    out[pair] = o;
+    if (S > ovf_limit) redo[atomicAdd(nredo, 1)] = pair;
 }
 
 __global__ void k_zero(int *p, int n)
@@ -447,11 +459,14 @@ __global__ void k_zero(int *p, int n)
     if (k < n) p[k] = 0;
 }
 
-// NW / SG epilogue, one wave per pair (src/sg.c:171-235, src/nw.c:155-157):
-// the lane candidates hold the last-column maximum over rows 0..m-2 with its
-// first row; the band row at `lastrow` holds the DP's last row as H-open,
-// from which the last-row maximum (first column on ties) and the corner
-// H[m-1][n-1] follow.
+// NW / SG epilogue, one wave per job (src/sg.c:171-235, src/nw.c:155-157).
+// A job's halves have adjacent refs (half 0, then half 1 when present) and
+// share the band row at `lastrow`, which holds the DP's last row as H-open
+// for both: the wave of the half-0 ref settles both pairs, the half-1
+// ref's wave has nothing to do.  The lane candidates hold each half's
+// last-column maximum over rows 0..m-2 with its first row; the last row is
+// scanned (its maximum, first column on ties) only when its end gap is
+// free (SG with s2_end); otherwise the corner H[m-1][n-1] is all it needs.
 __global__ void __launch_bounds__(256) k_sg_reduce(const int4 *cand, const int2 *bandbuf,
                                                    const PairRef *refs, int n, int alg, int flags,
                                                    int open, phip_out_t *out)
@@ -459,32 +474,56 @@ __global__ void __launch_bounds__(256) k_sg_reduce(const int4 *cand, const int2 
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (k >= n) return;
-    const PairRef r = refs[k];
-    int cm = PHIP_NEG_INF32, ci = 0;
-    if (lane < r.T) {
-        const int4 v = cand[r.cdoff + lane];
-        cm = v.x;
-        ci = v.y;
-    }
-    int rm = PHIP_NEG_INF32, rj = 0;
-    const int2 *row = bandbuf + r.lastrow;
-    for (int j = lane; j < r.n; j += 64) {
-        const int x = row[j].x;
-        const int v = (r.half ? hi16(x) : lo16(x)) + open;
-        if (v > rm) { rm = v; rj = j; }
-    }
-    for (int d = 32; d >= 1; d >>= 1) {
-        const int ocm = __shfl_xor(cm, d), oci = __shfl_xor(ci, d);
-        if (ocm > cm || (ocm == cm && oci < ci)) { cm = ocm; ci = oci; }
-        const int orm = __shfl_xor(rm, d), orj = __shfl_xor(rj, d);
-        if (orm > rm || (orm == rm && orj < rj)) { rm = orm; rj = orj; }
-    }
-    if (lane != 0) return;
-    const int x = row[r.n - 1].x;
-    const int corner = (r.half ? hi16(x) : lo16(x)) + open;
-    const int m = r.m, nn = r.n;
+    const PairRef r0 = refs[k];
+    if (r0.half) return;
+    const bool two = k + 1 < n && refs[k + 1].half;
+    const PairRef r1 = two ? refs[k + 1] : r0;
     const bool s1e = alg == PHIP_SG && (flags & PHIP_S1_END);
     const bool s2e = alg == PHIP_SG && (flags & PHIP_S2_END);
+    // last-column candidates, per half
+    int cm0 = PHIP_NEG_INF32, ci0 = 0, cm1 = PHIP_NEG_INF32, ci1 = 0;
+    if (s1e && lane < r0.T) {
+        const int4 v = cand[r0.cdoff + lane];
+        cm0 = v.x;
+        ci0 = v.y;
+        if (two) {
+            const int4 w = cand[r1.cdoff + lane];
+            cm1 = w.x;
+            ci1 = w.y;
+        }
+    }
+    // last row: one pass over the shared band row serves both halves
+    const int2 *row = bandbuf + r0.lastrow;
+    int rm0 = PHIP_NEG_INF32, rj0 = 0, rm1 = PHIP_NEG_INF32, rj1 = 0;
+    if (s2e) {
+        const int nmax = two ? imax(r0.n, r1.n) : r0.n;
+        for (int j = lane; j < nmax; j += 64) {
+            const int x = row[j].x;
+            const int v0 = lo16(x) + open, v1 = hi16(x) + open;
+            if (j < r0.n && v0 > rm0) { rm0 = v0; rj0 = j; }
+            if (two && j < r1.n && v1 > rm1) { rm1 = v1; rj1 = j; }
+        }
+    }
+    if (s1e || s2e) {
+        for (int d = 32; d >= 1; d >>= 1) {
+            int ov = __shfl_xor(cm0, d), oi = __shfl_xor(ci0, d);
+            if (ov > cm0 || (ov == cm0 && oi < ci0)) { cm0 = ov; ci0 = oi; }
+            ov = __shfl_xor(cm1, d); oi = __shfl_xor(ci1, d);
+            if (ov > cm1 || (ov == cm1 && oi < ci1)) { cm1 = ov; ci1 = oi; }
+            ov = __shfl_xor(rm0, d); oi = __shfl_xor(rj0, d);
+            if (ov > rm0 || (ov == rm0 && oi < rj0)) { rm0 = ov; rj0 = oi; }
+            ov = __shfl_xor(rm1, d); oi = __shfl_xor(rj1, d);
+            if (ov > rm1 || (ov == rm1 && oi < rj1)) { rm1 = ov; rj1 = oi; }
+        }
+    }
+    if (lane > 1 || (lane == 1 && !two)) return;
+    // lane 0 settles half 0, lane 1 half 1
+    const PairRef &r = lane ? r1 : r0;
+    const int cm = lane ? cm1 : cm0, ci = lane ? ci1 : ci0;
+    const int rm = lane ? rm1 : rm0, rj = lane ? rj1 : rj0;
+    const int x = row[r.n - 1].x;
+    const int corner = (lane ? hi16(x) : lo16(x)) + open;
+    const int m = r.m, nn = r.n;
     int score = corner, eq = m - 1, er = nn - 1;
     if (s1e && s2e) {
         const int S = imax(cm, rm);
@@ -941,7 +980,7 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream)
         HIP_OK(hipEventRecord(g_kw1, stream));
         g_kw_recorded = true;
         if (P->sw) {
-            hipLaunchKernelGGL(k_sw_reduce, dim3((P->nfast + 255) / 256), dim3(256), 0, stream,
+            hipLaunchKernelGGL(k_sw_reduce, dim3((P->nfast + 15) / 16), dim3(256), 0, stream,
                                P->dcd, P->drefs, P->nfast, P->ovf_limit, dp.out, P->dredo,
                                P->dcount);
         } else {
