@@ -310,6 +310,35 @@ fail:
     return NULL;
 }
 
+/* the device's 16-byte results into the caller's arrays, with the width
+ * flag of each pair */
+typedef struct {
+    const phip_out4_t *res;
+    parasail_hip_batch_result_t *out;
+    const parasail_matrix_t *matrix;
+    int alg, open, f0;
+} unpack_ctx_t;
+
+static void unpack_range(void *ctx, int lo, int hi)
+{
+    const unpack_ctx_t *c = (const unpack_ctx_t *)ctx;
+    int *score = c->out->score, *eq = c->out->end_query, *er = c->out->end_ref, *fl = c->out->flag;
+    for (int k = lo; k < hi; ++k) {
+        const phip_out4_t *o = &c->res[k];
+        int w = (o->status & PHIP_ST_W32) ? 32 : 16;
+        if (c->alg == PHIP_SW && w == 32) {
+            phip_out_t full;
+            memset(&full, 0, sizeof(full));
+            full.score = o->score; /* the SW width test reads the score only */
+            if (!phost_saturates(c->alg, 16, c->open, c->matrix, &full)) w = 16;
+        }
+        if (score) score[k] = o->score;
+        if (eq) eq[k] = o->end_query;
+        if (er) er[k] = o->end_ref;
+        if (fl) fl[k] = c->f0 | (w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32);
+    }
+}
+
 PHOST_EXPORT int parasail_hip_batch_align(parasail_hip_batch_t *hb, const char *alg_name, int open,
                                           int gap, parasail_hip_batch_result_t *out)
 {
@@ -334,21 +363,8 @@ PHOST_EXPORT int parasail_hip_batch_align(parasail_hip_batch_t *hb, const char *
         if (flags & PHIP_S2_END) f0 |= PARASAIL_FLAG_SG_S2_END;
     }
     f0 |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
-    for (int k = 0; k < hb->npairs; ++k) {
-        const phip_out4_t *o = &res[k];
-        int w = (o->status & PHIP_ST_W32) ? 32 : 16;
-        if (alg == PHIP_SW && w == 32) {
-            phip_out_t full;
-            memset(&full, 0, sizeof(full));
-            full.score = o->score;  /* the SW width test reads the score only */
-            if (!phost_saturates(alg, 16, open, hb->matrix, &full)) w = 16;
-        }
-        const int f = f0 | (w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32);
-        if (out->score) out->score[k] = o->score;
-        if (out->end_query) out->end_query[k] = o->end_query;
-        if (out->end_ref) out->end_ref[k] = o->end_ref;
-        if (out->flag) out->flag[k] = f;
-    }
+    unpack_ctx_t c = {res, out, hb->matrix, alg, open, f0};
+    phost_parallel_for(hb->npairs, 50000, unpack_range, &c);
     return 0;
 }
 

@@ -39,6 +39,10 @@ typedef struct {
     uint8_t *codes, *upper;
 } phost_enc_t;
 void phost_encode(const phost_enc_t *e, int npairs);
+/* fn(ctx, lo, hi) over [0, n) split across host threads, each taking at
+ * least min_per_thread items (one thread, the caller's, below that) */
+void phost_parallel_for(int n, int min_per_thread, void (*fn)(void *ctx, int lo, int hi),
+                        void *ctx);
 
 int phost_saturates(int alg, int width, int open, const parasail_matrix_t *matrix,
                     const phip_out_t *o);

@@ -1,6 +1,7 @@
 /* par.c -- host threads for the batch layer's byte work: turning residues
- * into matrix codes (and upper-cased copies for the CIGAR walk) costs more
- * than the kernels of a large batch when one thread does it. */
+ * into matrix codes (and upper-cased copies for the CIGAR walk), and the
+ * per-pair results into the caller's arrays, cost more than the kernels of
+ * a large batch (or a noticeable share of them) when one thread does it. */
 #include <ctype.h>
 #include <pthread.h>
 #include <unistd.h>
@@ -71,5 +72,49 @@ void phost_encode(const phost_enc_t *e, int npairs)
             pthread_join(th[i], NULL);
         else
             encode_range(e, part[i].lo, part[i].hi);
+    }
+}
+
+typedef struct {
+    void (*fn)(void *ctx, int lo, int hi);
+    void *ctx;
+    int lo, hi;
+} for_part_t;
+
+static void *for_part(void *arg)
+{
+    const for_part_t *p = (const for_part_t *)arg;
+    p->fn(p->ctx, p->lo, p->hi);
+    return NULL;
+}
+
+void phost_parallel_for(int n, int min_per_thread, void (*fn)(void *ctx, int lo, int hi),
+                        void *ctx)
+{
+    long nt = sysconf(_SC_NPROCESSORS_ONLN);
+    if (nt > PHOST_MAX_THREADS) nt = PHOST_MAX_THREADS;
+    if (min_per_thread < 1) min_per_thread = 1;
+    if (nt > n / min_per_thread) nt = n / min_per_thread;
+    if (nt <= 1) {
+        if (n > 0) fn(ctx, 0, n);
+        return;
+    }
+    pthread_t th[PHOST_MAX_THREADS];
+    for_part_t part[PHOST_MAX_THREADS];
+    int started[PHOST_MAX_THREADS] = {0};
+    for (long i = 0; i < nt; ++i) {
+        part[i].fn = fn;
+        part[i].ctx = ctx;
+        part[i].lo = (int)((int64_t)n * i / nt);
+        part[i].hi = (int)((int64_t)n * (i + 1) / nt);
+    }
+    for (long i = 1; i < nt; ++i)
+        started[i] = pthread_create(&th[i], NULL, for_part, &part[i]) == 0;
+    fn(ctx, part[0].lo, part[0].hi);
+    for (long i = 1; i < nt; ++i) {
+        if (started[i])
+            pthread_join(th[i], NULL);
+        else
+            fn(ctx, part[i].lo, part[i].hi);
     }
 }
