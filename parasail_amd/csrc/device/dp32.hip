@@ -515,7 +515,7 @@ size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 // fine: any R of kFineRs (no traceback, whose stores take 4 rows at a time)
 const int kFineRs[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20};
 
-bool shape(int m, int &T, int &R, int min_lanes = 16, bool fine = false)
+bool shape_calc(int m, int &T, int &R, int min_lanes, bool fine)
 {
     for (int tt : {16, 32, 64}) {
         if (tt < min_lanes) continue;
@@ -530,6 +530,31 @@ bool shape(int m, int &T, int &R, int min_lanes = 16, bool fine = false)
         }
     }
     return false;
+}
+
+// shape_calc, with the default lanes and rows looked up per query length
+// (the planners ask once per pair of large batches)
+bool shape(int m, int &T, int &R, int min_lanes = 16, bool fine = false)
+{
+    constexpr int kMaxM = 64 * 20;
+    struct Table {
+        int16_t t[kMaxM + 1], r[kMaxM + 1];
+        Table()
+        {
+            for (int mm = 0; mm <= kMaxM; ++mm) {
+                int tt = 0, rr = 0;
+                if (mm < 1 || !shape_calc(mm, tt, rr, 16, false)) tt = rr = 0;
+                t[mm] = (int16_t)tt;
+                r[mm] = (int16_t)rr;
+            }
+        }
+    };
+    if (min_lanes != 16 || fine || m < 1 || m > kMaxM) return shape_calc(m, T, R, min_lanes, fine);
+    static const Table tab;
+    if (!tab.t[m]) return false;
+    T = tab.t[m];
+    R = tab.r[m];
+    return true;
 }
 
 }  // namespace
@@ -593,12 +618,36 @@ bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t
         keys.push_back(k);
     }
     // jobs of one shape share waves; longest first so a wave's jobs end
-    // together and the longest waves start first
-    std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
-        if (a.T != b.T) return a.T > b.T;
-        if (a.R != b.R) return a.R > b.R;
-        return a.n > b.n;
-    });
+    // together and the longest waves start first: (T, R, n) descending, by
+    // a stable radix sort of 32-bit keys (a comparison sort of a large batch,
+    // most of whose keys tie, costs milliseconds)
+    {
+        const size_t N = keys.size();
+        std::vector<uint32_t> k(N), k2(N);
+        std::vector<int> ix(N), ix2(N);
+        for (size_t i = 0; i < N; ++i) {
+            const int tr = keys[i].T == 64 ? 0 : keys[i].T == 32 ? 1 : 2;
+            const uint32_t rank = (uint32_t)(tr * 64 + (63 - std::min(keys[i].R, 63)));
+            const uint32_t nn = (uint32_t)std::min(keys[i].n, 0xFFFFFF);
+            k[i] = (rank << 24) | (0xFFFFFFu - nn);
+            ix[i] = (int)i;
+        }
+        for (int shift = 0; shift < 32; shift += 8) {
+            size_t cnt[257] = {0};
+            for (size_t i = 0; i < N; ++i) ++cnt[((k[i] >> shift) & 255) + 1];
+            for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+            for (size_t i = 0; i < N; ++i) {
+                const size_t pos = cnt[(k[i] >> shift) & 255]++;
+                k2[pos] = k[i];
+                ix2[pos] = ix[i];
+            }
+            k.swap(k2);
+            ix.swap(ix2);
+        }
+        std::vector<Key> sorted(N);
+        for (size_t i = 0; i < N; ++i) sorted[i] = keys[ix[i]];
+        keys.swap(sorted);
+    }
     std::vector<Dp32Job> jobs;
     std::vector<Dp32Wave> waves;
     for (size_t x = 0; x < keys.size();) {

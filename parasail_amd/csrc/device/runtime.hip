@@ -232,25 +232,34 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
         delete r;
         return nullptr;
     }
-    // results come back through pinned memory (a pageable copy of a large
-    // batch's results costs more than some of its kernels)
-    if (hipHostMalloc((void **)&r->hout, (size_t)np * sizeof(phip_out_t), hipHostMallocDefault) !=
-        hipSuccess) {
+    return r;
+}
+
+// Pinned host copies of the results, for the batches the API keeps resident
+// (a pageable copy of a large batch's results costs more than some of its
+// kernels); a one-call batch copies straight into the caller's array.
+static int resident_host_buffers(phip_resident_t *r)
+{
+    const int np = r->np;
+    if (!r->hout &&
+        hipHostMalloc((void **)&r->hout, (size_t)np * sizeof(phip_out_t), hipHostMallocDefault) !=
+            hipSuccess) {
         (void)hipGetLastError();
         r->hout = nullptr;
     }
-    r->hout4_pinned = hipHostMalloc((void **)&r->hout4, (size_t)np * sizeof(phip_out4_t),
-                                    hipHostMallocDefault) == hipSuccess;
-    if (!r->hout4_pinned) {
-        (void)hipGetLastError();
-        r->hout4 = (phip_out4_t *)malloc((size_t)np * sizeof(phip_out4_t));
+    if (!r->hout4) {
+        r->hout4_pinned = hipHostMalloc((void **)&r->hout4, (size_t)np * sizeof(phip_out4_t),
+                                        hipHostMallocDefault) == hipSuccess;
+        if (!r->hout4_pinned) {
+            (void)hipGetLastError();
+            r->hout4 = (phip_out4_t *)malloc((size_t)np * sizeof(phip_out4_t));
+        }
     }
     if (!r->hout4) {
         phip_set_error("resident batch: out of host memory");
-        resident_free(r);
-        return nullptr;
+        return -1;
     }
-    return r;
+    return 0;
 }
 
 static void resident_free(phip_resident_t *r)
@@ -297,6 +306,36 @@ static DPParams make_params(const phip_resident_t *r, int alg, int kind, int sg_
 }
 
 static int simple_run(phip_resident_t *r, const DPParams &dp, const int32_t *list, int count);
+
+// Pinned bounce buffer for the planner's uploads (held under g_lock; each
+// use waits for its copies before returning): a pageable copy is staged by
+// the runtime in pieces and costs ~1 ms for a plan of 100,000 pairs.
+static char *g_up = nullptr;
+static size_t g_up_cap = 0;
+
+static int upload_via_pinned(void *dst, const void *src, size_t bytes, size_t at)
+{
+    if (!bytes) return 0;
+    if (at + bytes > g_up_cap) return -1;
+    memcpy(g_up + at, src, bytes);
+    return hipMemcpyAsync(dst, g_up + at, bytes, hipMemcpyHostToDevice, g_stream) == hipSuccess ? 0 : -1;
+}
+
+static int reserve_up(size_t bytes)
+{
+    if (bytes <= g_up_cap) return 0;
+    if (g_up) (void)hipHostFree(g_up);
+    g_up = nullptr;
+    g_up_cap = 0;
+    const size_t cap = std::max(bytes + (bytes >> 2), (size_t)4 << 20);
+    if (hipHostMalloc((void **)&g_up, cap, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        g_up = nullptr;
+        return -1;
+    }
+    g_up_cap = cap;
+    return 0;
+}
 
 // Runs one alignment pass over the resident batch; results stay in r->dout.
 static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int open, int gap,
@@ -357,14 +396,25 @@ static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int
             if (r->dplan.bytes) {
                 r->dplan_mem = (char *)dev_alloc(r->dplan.bytes);
                 if (!r->dplan_mem) return -1;
-                HIP_OK(hipMemcpyAsync(r->dplan_mem, r->dplan.image.data(), r->dplan.bytes,
+            }
+            const size_t a_slow = al256(r->dplan.bytes), a_tlay = a_slow + al256(r->slow.size() * 4);
+            if (reserve_up(a_tlay + (size_t)r->np * 4) == 0) {
+                if (upload_via_pinned(r->dplan_mem, r->dplan.image.data(), r->dplan.bytes, 0) != 0 ||
+                    upload_via_pinned(r->dslow, r->slow.data(), r->slow.size() * 4, a_slow) != 0 ||
+                    upload_via_pinned(r->dtlay, r->tlay.data(), (size_t)r->np * 4, a_tlay) != 0) {
+                    phip_set_error("dp32 plan upload failed");
+                    return -1;
+                }
+            } else {
+                if (r->dplan.bytes)
+                    HIP_OK(hipMemcpyAsync(r->dplan_mem, r->dplan.image.data(), r->dplan.bytes,
+                                          hipMemcpyHostToDevice, g_stream));
+                if (!r->slow.empty())
+                    HIP_OK(hipMemcpyAsync(r->dslow, r->slow.data(), r->slow.size() * 4,
+                                          hipMemcpyHostToDevice, g_stream));
+                HIP_OK(hipMemcpyAsync(r->dtlay, r->tlay.data(), (size_t)r->np * 4,
                                       hipMemcpyHostToDevice, g_stream));
             }
-            if (!r->slow.empty())
-                HIP_OK(hipMemcpyAsync(r->dslow, r->slow.data(), r->slow.size() * 4,
-                                      hipMemcpyHostToDevice, g_stream));
-            HIP_OK(hipMemcpyAsync(r->dtlay, r->tlay.data(), (size_t)r->np * 4,
-                                  hipMemcpyHostToDevice, g_stream));
             // the pageable host vectors stay alive in r; the copies must end
             // before they change
             HIP_OK(hipStreamSynchronize(g_stream));
@@ -696,8 +746,51 @@ extern "C" void *phip_resident_create(const phip_batch_t *b)
     if (!b || b->npairs <= 0) { phip_set_error("phip_resident_create: bad arguments"); return nullptr; }
     pthread_mutex_lock(&g_lock);
     phip_resident_t *r = resident_create(b);
+    if (r && resident_host_buffers(r) != 0) {
+        resident_free(r);
+        r = nullptr;
+    }
     pthread_mutex_unlock(&g_lock);
     return r;
+}
+
+// ---------------------------------------------------------------- staging
+
+namespace {
+// One pinned host buffer for the per-call batch entry points: residues are
+// encoded straight into it and uploaded by DMA from there (a pageable
+// upload is staged by the runtime in small pieces, and fresh host buffers
+// cost a page fault per page).  It grows to the largest batch seen, up to
+// kStageMax, and is kept for the process; one caller holds it at a time.
+constexpr size_t kStageMax = (size_t)1 << 30;
+pthread_mutex_t g_stage_mu = PTHREAD_MUTEX_INITIALIZER;
+char *g_stage = nullptr;
+size_t g_stage_cap = 0;
+}  // namespace
+
+extern "C" void *phip_stage_acquire(size_t bytes)
+{
+    if (bytes == 0 || bytes > kStageMax || phip_init() != 0) return nullptr;
+    pthread_mutex_lock(&g_stage_mu);
+    if (bytes > g_stage_cap) {
+        const size_t cap = std::min(kStageMax, std::max(bytes + (bytes >> 2), (size_t)16 << 20));
+        if (g_stage) (void)hipHostFree(g_stage);
+        g_stage = nullptr;
+        g_stage_cap = 0;
+        if (hipHostMalloc((void **)&g_stage, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            g_stage = nullptr;
+            pthread_mutex_unlock(&g_stage_mu);
+            return nullptr;
+        }
+        g_stage_cap = cap;
+    }
+    return g_stage;
+}
+
+extern "C" void phip_stage_release(void *p)
+{
+    if (p) pthread_mutex_unlock(&g_stage_mu);
 }
 
 __global__ void k_pack_out4(const phip_out_t *out, phip_out4_t *o4, int n)
@@ -714,7 +807,8 @@ extern "C" int phip_resident_align4(void *handle, int alg, int sg_flags, int ope
     phip_resident_t *r = (phip_resident_t *)handle;
     if (!r || !res) { phip_set_error("phip_resident_align4: bad arguments"); return -1; }
     pthread_mutex_lock(&g_lock);
-    int rc = resident_run(r, alg, PHIP_SCORE, sg_flags, open, gap, 0);
+    int rc = resident_host_buffers(r);
+    if (rc == 0) rc = resident_run(r, alg, PHIP_SCORE, sg_flags, open, gap, 0);
     do {
         if (rc != 0) break;
         rc = -1;

@@ -99,27 +99,48 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
         ttot += tl[k];
     }
     {
-        uint8_t *q = s1 ? (uint8_t *)malloc((size_t)qtot + 1) : NULL;
-        uint8_t *t = (uint8_t *)malloc((size_t)ttot + 1);
         /* CIGARs: traceback and walk on the device, from the residues as
          * given, upper-cased (cigar_template.c compares them) */
-        uint8_t *qraw = NULL, *traw = NULL;
+        const int dev_cigar = kind == PHIP_TRACE && out->cigars;
+        /* the codes (and upper-cased residues) go to the runtime's pinned
+         * staging buffer when it can be had, else to ordinary memory */
+        const size_t sq = ((size_t)(s1 ? qtot + 1 : 0) + 255) & ~(size_t)255;
+        const size_t st = ((size_t)ttot + 1 + 255) & ~(size_t)255;
+        uint8_t *stage = (uint8_t *)phip_stage_acquire((sq + st) * (dev_cigar ? 2 : 1));
+        uint8_t *q = NULL, *t = NULL, *qraw = NULL, *traw = NULL;
+        if (stage) {
+            q = s1 ? stage : NULL;
+            t = stage + sq;
+            if (dev_cigar) {
+                qraw = stage + sq + st;
+                traw = qraw + sq;
+            }
+        } else {
+            q = s1 ? (uint8_t *)malloc((size_t)qtot + 1) : NULL;
+            t = (uint8_t *)malloc((size_t)ttot + 1);
+            if (dev_cigar) {
+                qraw = (uint8_t *)malloc((size_t)qtot + 1);
+                traw = (uint8_t *)malloc((size_t)ttot + 1);
+            }
+        }
         phip_cigars_t cg;
         memset(&cg, 0, sizeof(cg));
-        const int dev_cigar = kind == PHIP_TRACE && out->cigars;
-        if ((s1 && !q) || !t) { free(q); free(t); goto oom; }
+        if ((s1 && !q) || !t || (dev_cigar && (!qraw || !traw))) {
+            if (stage) phip_stage_release(stage);
+            else { free(q); free(t); free(qraw); free(traw); }
+            goto oom;
+        }
         if (dev_cigar) {
             int64_t otot = 0;
             for (int k = 0; k < npairs; ++k) otot += (int64_t)ql[k] + tl[k] + 2;
-            qraw = (uint8_t *)malloc((size_t)qtot + 1);
-            traw = (uint8_t *)malloc((size_t)ttot + 1);
             cg.ops = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(otot + 1));
             cg.off = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
             cg.nops = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
             cg.beg = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)npairs);
-            if (!qraw || !traw || !cg.ops || !cg.off || !cg.nops || !cg.beg) {
-                free(qraw); free(traw); free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
-                free(q); free(t);
+            if (!cg.ops || !cg.off || !cg.nops || !cg.beg) {
+                free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
+                if (stage) phip_stage_release(stage);
+                else { free(q); free(t); free(qraw); free(traw); }
                 goto oom;
             }
         }
@@ -154,10 +175,8 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
         b.toff = to;
         b.tlen = tl;
         int rc = dev_cigar ? phip_align_cigars(&b, qraw, traw, res, &cg) : phip_align(&b, res, NULL);
-        free(q);
-        free(t);
-        free(qraw);
-        free(traw);
+        if (stage) phip_stage_release(stage);
+        else { free(q); free(t); free(qraw); free(traw); }
         if (rc != 0) {
             snprintf(g_err, sizeof(g_err), "%s", phip_last_error());
             fprintf(stderr, "%s: %s\n", __func__, g_err);
@@ -401,7 +420,7 @@ PHOST_EXPORT int parasail_hip_nw_banded_batch(const char *s1, const int64_t *s1_
     int64_t *qo = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
     int64_t *to = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
     phip_out_t *res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)npairs);
-    uint8_t *q = NULL, *t = NULL;
+    uint8_t *q = NULL, *t = NULL, *stage = NULL;
     int rc = -1;
     if (!ql || !tl || !qo || !to || !res) goto done;
     for (int p = 0; p < npairs; ++p) {
@@ -420,8 +439,17 @@ PHOST_EXPORT int parasail_hip_nw_banded_batch(const char *s1, const int64_t *s1_
         qtot += ql[p];
         ttot += tl[p];
     }
-    q = (uint8_t *)malloc((size_t)qtot + 1);
-    t = (uint8_t *)malloc((size_t)ttot + 1);
+    {
+        const size_t sq = ((size_t)qtot + 1 + 255) & ~(size_t)255;
+        stage = (uint8_t *)phip_stage_acquire(sq + (size_t)ttot + 1);
+        if (stage) {
+            q = stage;
+            t = stage + sq;
+        } else {
+            q = (uint8_t *)malloc((size_t)qtot + 1);
+            t = (uint8_t *)malloc((size_t)ttot + 1);
+        }
+    }
     if (!q || !t) goto done;
     {
         phost_enc_t e = {s1, s1_off, ql, qo, matrix->mapper, q, NULL};
@@ -468,6 +496,8 @@ PHOST_EXPORT int parasail_hip_nw_banded_batch(const char *s1, const int64_t *s1_
     }
     rc = 0;
 done:
-    free(ql); free(tl); free(qo); free(to); free(res); free(q); free(t);
+    if (stage) phip_stage_release(stage);
+    else { free(q); free(t); }
+    free(ql); free(tl); free(qo); free(to); free(res);
     return rc;
 }

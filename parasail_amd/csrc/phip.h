@@ -97,6 +97,11 @@ int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, const uint8_t 
 /* Device-resident batch: residues, lengths and results stay in HBM across
  * calls, and the wavefront plan for each scoring is built once.  Score-only
  * alignments; NULL / -1 on failure. */
+/* The process's pinned staging buffer of at least `bytes` (NULL when it
+ * cannot be had: use ordinary memory).  The caller holds it until
+ * phip_stage_release; other callers wait meanwhile. */
+void *phip_stage_acquire(size_t bytes);
+void phip_stage_release(void *p);
 void *phip_resident_create(const phip_batch_t *b);
 /* Score-only results of a resident batch, 16 bytes a pair. */
 typedef struct phip_out4 {
