@@ -202,9 +202,10 @@ def stats_trace(peak32, npairs=100000):
     b = WL.config2(npairs=npairs)
     log("stats/trace: %d pairs, %.3e cells (generated in %.1f s)"
         % (len(b), b.cells, time.perf_counter() - t0))
-    small = b.subset(np.arange(min(64, len(b))))
-    P.align_arrays("sg", 1, 0, small, 5, 2, dm)
-    P.align_arrays("sg", 2, 0, small, 5, 2, dm, want_cigar=True)
+    # one untimed call of each at full size first: kernels loaded, and the
+    # library's pinned staging buffer grown to the batch (kept for the process)
+    P.align_arrays("sg", 1, 0, b, 5, 2, dm)
+    P.align_arrays("sg", 2, 0, b, 5, 2, dm, want_cigar=True)
     rs = P.align_arrays("sg", 1, 0, b, 5, 2, dm)
     wall_s = P.last_call_ms() / 1e3
     kms_s = P.last_kernel_ms()

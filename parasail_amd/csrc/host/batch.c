@@ -43,6 +43,73 @@ static int parse_alg(const char *name, int *alg, int *flags)
 
 static char g_err[256];
 
+/* the device results of [lo, hi) into the caller's arrays and CIGARs, with
+ * the flags and saturation the reference's functions report */
+typedef struct {
+    const phip_out_t *res;
+    parasail_hip_batch_result_t *out;
+    const parasail_matrix_t *matrix;
+    const phip_cigars_t *cg;
+    int alg, flags, kind, width, open, dev_cigar;
+} convert_ctx_t;
+
+static void convert_range(void *ctx, int lo, int hi)
+{
+    const convert_ctx_t *x = (const convert_ctx_t *)ctx;
+    const int alg = x->alg, flags = x->flags, kind = x->kind, width = x->width, open = x->open;
+    const parasail_matrix_t *matrix = x->matrix;
+    parasail_hip_batch_result_t *out = x->out;
+    for (int k = lo; k < hi; ++k) {
+        const phip_out_t *o = &x->res[k];
+        int w = width, sat = 0;
+        if (w == 0) {
+            w = (o->status & PHIP_ST_W32) ? 32 : 16;
+            if (alg == PHIP_SW && w == 32 && !phost_saturates(alg, 16, open, matrix, o)) w = 16;
+        } else {
+            sat = phost_saturates(alg, w, open, matrix, o);
+        }
+        int f = alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
+        if (alg == PHIP_SG) {
+            if (flags & PHIP_S1_BEG) f |= PARASAIL_FLAG_SG_S1_BEG;
+            if (flags & PHIP_S1_END) f |= PARASAIL_FLAG_SG_S1_END;
+            if (flags & PHIP_S2_BEG) f |= PARASAIL_FLAG_SG_S2_BEG;
+            if (flags & PHIP_S2_END) f |= PARASAIL_FLAG_SG_S2_END;
+        }
+        f |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
+        f |= w == 8 ? PARASAIL_FLAG_BITS_8 : w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32;
+        if (kind == PHIP_STATS) f |= PARASAIL_FLAG_STATS;
+        if (kind == PHIP_TRACE) f |= PARASAIL_FLAG_TRACE;
+        if (sat) f |= PARASAIL_FLAG_SATURATED;
+        if (out->score) out->score[k] = sat ? (alg == PHIP_SW ? (int)((1L << w) - 1) : 0) : o->score;
+        if (out->end_query) out->end_query[k] = sat ? 0 : o->end_query;
+        if (out->end_ref) out->end_ref[k] = sat ? 0 : o->end_ref;
+        if (out->matches) out->matches[k] = sat ? 0 : o->matches;
+        if (out->similar) out->similar[k] = sat ? 0 : o->similar;
+        if (out->length) out->length[k] = sat ? 0 : o->length;
+        if (out->flag) out->flag[k] = f;
+        if (x->dev_cigar) {
+            const phip_cigars_t *cg = x->cg;
+            parasail_cigar_t *c = NULL;
+            if (!sat && cg->nops[k] >= 0) {
+                c = (parasail_cigar_t *)malloc(sizeof(parasail_cigar_t));
+                uint32_t *seq = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(cg->nops[k] + 1));
+                if (!c || !seq) {
+                    free(c);
+                    free(seq);
+                    c = NULL;
+                } else {
+                    memcpy(seq, cg->ops + cg->off[k], sizeof(uint32_t) * (size_t)cg->nops[k]);
+                    c->seq = seq;
+                    c->len = cg->nops[k];
+                    c->beg_query = cg->beg[2 * k];
+                    c->beg_ref = cg->beg[2 * k + 1];
+                }
+            }
+            out->cigars[k] = c;
+        }
+    }
+}
+
 PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int width,
                                           const char *s1, const int64_t *s1_off,
                                           const int *s1_len, const char *s2,
@@ -184,53 +251,10 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
             free(ql); free(tl); free(qo); free(to); free(res);
             return -1;
         }
-        for (int k = 0; k < npairs; ++k) {
-            const phip_out_t *o = &res[k];
-            int w = width, sat = 0;
-            if (w == 0) {
-                w = (o->status & PHIP_ST_W32) ? 32 : 16;
-                if (alg == PHIP_SW && w == 32 && !phost_saturates(alg, 16, open, matrix, o)) w = 16;
-            } else {
-                sat = phost_saturates(alg, w, open, matrix, o);
-            }
-            int f = alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
-            if (alg == PHIP_SG) {
-                if (flags & PHIP_S1_BEG) f |= PARASAIL_FLAG_SG_S1_BEG;
-                if (flags & PHIP_S1_END) f |= PARASAIL_FLAG_SG_S1_END;
-                if (flags & PHIP_S2_BEG) f |= PARASAIL_FLAG_SG_S2_BEG;
-                if (flags & PHIP_S2_END) f |= PARASAIL_FLAG_SG_S2_END;
-            }
-            f |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
-            f |= w == 8 ? PARASAIL_FLAG_BITS_8 : w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32;
-            if (kind == PHIP_STATS) f |= PARASAIL_FLAG_STATS;
-            if (kind == PHIP_TRACE) f |= PARASAIL_FLAG_TRACE;
-            if (sat) f |= PARASAIL_FLAG_SATURATED;
-            if (out->score) out->score[k] = sat ? (alg == PHIP_SW ? (int)((1L << w) - 1) : 0) : o->score;
-            if (out->end_query) out->end_query[k] = sat ? 0 : o->end_query;
-            if (out->end_ref) out->end_ref[k] = sat ? 0 : o->end_ref;
-            if (out->matches) out->matches[k] = sat ? 0 : o->matches;
-            if (out->similar) out->similar[k] = sat ? 0 : o->similar;
-            if (out->length) out->length[k] = sat ? 0 : o->length;
-            if (out->flag) out->flag[k] = f;
-            if (dev_cigar) {
-                parasail_cigar_t *c = NULL;
-                if (!sat && cg.nops[k] >= 0) {
-                    c = (parasail_cigar_t *)malloc(sizeof(parasail_cigar_t));
-                    uint32_t *seq = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(cg.nops[k] + 1));
-                    if (!c || !seq) {
-                        free(c);
-                        free(seq);
-                        c = NULL;
-                    } else {
-                        memcpy(seq, cg.ops + cg.off[k], sizeof(uint32_t) * (size_t)cg.nops[k]);
-                        c->seq = seq;
-                        c->len = cg.nops[k];
-                        c->beg_query = cg.beg[2 * k];
-                        c->beg_ref = cg.beg[2 * k + 1];
-                    }
-                }
-                out->cigars[k] = c;
-            }
+        {
+            convert_ctx_t cc = {res, out, matrix, &cg, alg, flags, kind, width, open, dev_cigar};
+            /* the CIGAR objects are two mallocs each: several threads share them */
+            phost_parallel_for(npairs, dev_cigar ? 25000 : 50000, convert_range, &cc);
         }
         free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
     }
