@@ -664,9 +664,8 @@ extern "C" int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, con
     char *mem = nullptr;
     do {
         if (!r) break;
-        if (resident_run(r, b->alg, PHIP_TRACE, b->sg_flags, b->open, b->gap, b->want_sat_check) != 0)
-            break;
-        // raw residues (upper-cased by the caller) and the op buffers
+        // raw residues (upper-cased by the caller) and the op buffers; the
+        // uploads run while the host plans the DP below
         int64_t qtot = 0, ttot = 0, otot = 0;
         std::vector<int64_t> opoff(np);
         for (int i = 0; i < np; ++i) {
@@ -687,6 +686,8 @@ extern "C" int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, con
         ok &= hipMemcpyAsync(mem + o_oo, opoff.data(), (size_t)np * 8, hipMemcpyHostToDevice,
                              g_stream) == hipSuccess;
         if (!ok) { phip_set_error("phip_align_cigars: upload failed"); break; }
+        if (resident_run(r, b->alg, PHIP_TRACE, b->sg_flags, b->open, b->gap, b->want_sat_check) != 0)
+            break;
         CigarArgs a;
         a.qraw = (const uint8_t *)(mem + o_q);
         a.traw = (const uint8_t *)(mem + o_t);
@@ -708,8 +709,22 @@ extern "C" int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, con
         // the call's device time: DP kernels and the walk
         if (hipEventRecord(g_ev1, g_stream) != hipSuccess) break;
         if (resident_fetch(r, out, nullptr) != 0) break;
-        ok = hipMemcpy(cig->nops, a.nops, (size_t)np * 4, hipMemcpyDeviceToHost) == hipSuccess;
-        ok &= hipMemcpy(cig->beg, a.beg, (size_t)np * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        // downloads land in the pinned bounce buffer (DMA), then in the
+        // caller's arrays
+        const size_t a_beg = al256((size_t)np * 4);
+        if (reserve_up(a_beg + (size_t)np * 8) == 0) {
+            ok = hipMemcpyAsync(g_up, a.nops, (size_t)np * 4, hipMemcpyDeviceToHost, g_stream) == hipSuccess;
+            ok &= hipMemcpyAsync(g_up + a_beg, a.beg, (size_t)np * 8, hipMemcpyDeviceToHost, g_stream) ==
+                  hipSuccess;
+            ok &= hipStreamSynchronize(g_stream) == hipSuccess;
+            if (ok) {
+                memcpy(cig->nops, g_up, (size_t)np * 4);
+                memcpy(cig->beg, g_up + a_beg, (size_t)np * 8);
+            }
+        } else {
+            ok = hipMemcpy(cig->nops, a.nops, (size_t)np * 4, hipMemcpyDeviceToHost) == hipSuccess;
+            ok &= hipMemcpy(cig->beg, a.beg, (size_t)np * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        }
         if (!ok) { phip_set_error("phip_align_cigars: download failed"); break; }
         // the ops reserve m + n + 2 slots a pair and use a few: gather the
         // used ones back to back on the device and download only those
@@ -725,9 +740,11 @@ extern "C" int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, con
             ok = ok && hipMemcpyAsync(dco, cig->off, (size_t)np * 8, hipMemcpyHostToDevice, g_stream) ==
                            hipSuccess;
             ok = ok && cigar_compact_launch(a.ops, a.opoff, a.nops, dco, np, dcomp, g_stream) == 0;
-            ok = ok && hipMemcpyAsync(cig->ops, dcomp, (size_t)ctot * 4, hipMemcpyDeviceToHost,
-                                      g_stream) == hipSuccess;
+            const bool bounce = ok && reserve_up((size_t)ctot * 4) == 0;
+            ok = ok && hipMemcpyAsync(bounce ? (void *)g_up : (void *)cig->ops, dcomp,
+                                      (size_t)ctot * 4, hipMemcpyDeviceToHost, g_stream) == hipSuccess;
             ok = ok && hipStreamSynchronize(g_stream) == hipSuccess;
+            if (ok && bounce) memcpy(cig->ops, g_up, (size_t)ctot * 4);
             if (dco) dev_free(dco);
             if (dcomp) dev_free(dcomp);
             if (!ok) { phip_set_error("phip_align_cigars: download failed"); break; }
