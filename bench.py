@@ -47,6 +47,13 @@ def make_resident(batch, matrix):
     L.parasail_hip_batch_align.restype = ctypes.c_int
     L.parasail_hip_batch_align.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
                                            ctypes.c_int, ctypes.POINTER(P.BatchResult)]
+    L.parasail_hip_batch_align_multi.restype = ctypes.c_int
+    L.parasail_hip_batch_align_multi.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
+                                                 ctypes.c_int, ctypes.POINTER(P.BatchResult)]
+    L.parasail_hip_last_pass_ms.restype = ctypes.c_int
+    L.parasail_hip_last_pass_ms.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]
     L.parasail_hip_batch_free.argtypes = [ctypes.c_void_p]
     I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
     h = L.parasail_hip_batch_create(batch.q, batch.qoff.ctypes.data_as(I64),
@@ -69,6 +76,31 @@ def run_resident(h, alg, n, keep=None):
     if L.parasail_hip_batch_align(h, alg.encode(), 11, 1, ctypes.byref(br)) != 0:
         raise RuntimeError("parasail_hip_batch_align(%s) failed" % alg)
     return out, P.last_kernel_ms(), P.last_main_kernel_ms()
+
+
+class MultiPass:
+    """One parasail_hip_batch_align_multi call per step: every algorithm's
+    pass over the resident batch, each pass's results into its own arrays."""
+
+    def __init__(self, h, algs, bufs):
+        self.h, self.algs = h, algs
+        self.outs = (P.BatchResult * len(algs))()
+        for k, alg in enumerate(algs):
+            for key, v in bufs[alg].items():
+                setattr(self.outs[k], key, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        self.names = (ctypes.c_char_p * len(algs))(*[x.encode() for x in algs])
+
+    def run(self):
+        L = P.lib()
+        if L.parasail_hip_batch_align_multi(self.h, len(self.algs), self.names, 11, 1,
+                                            self.outs) != 0:
+            raise RuntimeError("parasail_hip_batch_align_multi failed")
+        ms = []
+        for k in range(len(self.algs)):
+            km, mm = ctypes.c_double(), ctypes.c_double()
+            L.parasail_hip_last_pass_ms(k, ctypes.byref(km), ctypes.byref(mm))
+            ms.append((km.value, mm.value))
+        return ms
 
 
 def per_call_gcups(reps=1):
@@ -384,16 +416,17 @@ def main():
     h = make_resident(batch, m)
     bufs = {alg: {k: np.zeros(len(batch), dtype=np.int32)
                   for k in ("score", "end_query", "end_ref", "flag")} for alg in algs}
+    # a step: SW, NW and SG over the resident batch in one library call, each
+    # pass's download and conversion overlapping the next pass's kernels
+    mp = MultiPass(h, algs, bufs)
     for w in range(a.warmup):
-        for alg in algs:
-            run_resident(h, alg, len(batch), bufs[alg])
+        mp.run()
         log("warmup %d done" % w)
     kms = {alg: [] for alg in algs}
     mms = {alg: [] for alg in algs}
     t0 = time.perf_counter()
     for s in range(a.steps):
-        for alg in algs:
-            _, ms, mm = run_resident(h, alg, len(batch), bufs[alg])
+        for alg, (ms, mm) in zip(algs, mp.run()):
             kms[alg].append(ms)
             mms[alg].append(mm)
         log("step %d done" % s)

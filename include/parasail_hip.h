@@ -404,6 +404,17 @@ parasail_hip_batch_t *parasail_hip_batch_create(const char *s1, const int64_t *s
                                                 int npairs, const parasail_matrix_t *matrix);
 int parasail_hip_batch_align(parasail_hip_batch_t *batch, const char *alg_name, int open,
                              int gap, parasail_hip_batch_result_t *out);
+/* Several algorithms (at most 4, e.g. "sw", "nw", "sg") over one resident
+ * batch in one call, results into out[0..nalg-1] as parasail_hip_batch_align
+ * gives them: every pass is launched before the first is converted, so the
+ * host's work on one pass's results overlaps the next passes' kernels.  No
+ * reference counterpart (a batch interface; the reference has none).
+ * parasail_hip_last_pass_ms reports pass k's device time of the last such
+ * call, all of its kernels and its main kernel alone.  0, or -1. */
+int parasail_hip_batch_align_multi(parasail_hip_batch_t *batch, int nalg,
+                                   const char *const *alg_names, int open, int gap,
+                                   parasail_hip_batch_result_t *out);
+int parasail_hip_last_pass_ms(int k, double *kernel_ms, double *main_kernel_ms);
 void parasail_hip_batch_free(parasail_hip_batch_t *batch);
 
 /* ---- database search and serving (no reference counterpart; the

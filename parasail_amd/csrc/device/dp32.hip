@@ -632,12 +632,13 @@ bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t
             k[i] = (rank << 24) | (0xFFFFFFu - nn);
             ix[i] = (int)i;
         }
-        for (int shift = 0; shift < 32; shift += 8) {
-            size_t cnt[257] = {0};
-            for (size_t i = 0; i < N; ++i) ++cnt[((k[i] >> shift) & 255) + 1];
-            for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+        std::vector<uint32_t> cnt(65537);
+        for (int shift = 0; shift < 32; shift += 16) {
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (size_t i = 0; i < N; ++i) ++cnt[((k[i] >> shift) & 0xFFFF) + 1];
+            for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
             for (size_t i = 0; i < N; ++i) {
-                const size_t pos = cnt[(k[i] >> shift) & 255]++;
+                const size_t pos = cnt[(k[i] >> shift) & 0xFFFF]++;
                 k2[pos] = k[i];
                 ix2[pos] = ix[i];
             }
@@ -650,6 +651,8 @@ bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t
     }
     std::vector<Dp32Job> jobs;
     std::vector<Dp32Wave> waves;
+    jobs.reserve(keys.size());
+    waves.reserve(keys.size() / 4 + 1);
     for (size_t x = 0; x < keys.size();) {
         const int per = 64 / keys[x].T;
         Dp32Wave w;

@@ -158,6 +158,13 @@ struct phip_resident {
     // score-only results packed to 16 bytes a pair (device, then pinned host)
     phip_out4_t *dout4 = nullptr, *hout4 = nullptr;
     bool hout4_pinned = false;
+    // several passes in one call (phip_resident_launch4): per pass a device
+    // and a pinned copy of the packed results and its events
+    static constexpr int kPasses = 4;
+    phip_out4_t *pd4[kPasses] = {}, *ph4[kPasses] = {};
+    hipEvent_t pev[kPasses][5] = {};  // start, end, main start, main end, downloaded
+    int npass = 0;
+    bool pass_wave[kPasses] = {};
 };
 
 typedef struct phip_resident phip_resident_t;
@@ -265,6 +272,12 @@ static int resident_host_buffers(phip_resident_t *r)
 static void resident_free(phip_resident_t *r)
 {
     if (!r) return;
+    for (int k = 0; k < phip_resident_t::kPasses; ++k) {
+        if (r->pd4[k]) dev_free(r->pd4[k]);
+        if (r->ph4[k]) (void)hipHostFree(r->ph4[k]);
+        for (int e = 0; e < 5; ++e)
+            if (r->pev[k][e]) (void)hipEventDestroy(r->pev[k][e]);
+    }
     for (WavePlan *p : r->plans) wave_plan_free(p);
     if (r->dplan_mem) dev_free(r->dplan_mem);
     dev_free(r->base);
@@ -339,7 +352,7 @@ static int reserve_up(size_t bytes)
 
 // Runs one alignment pass over the resident batch; results stay in r->dout.
 static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int open, int gap,
-                        int check)
+                        int check, hipEvent_t me0 = nullptr, hipEvent_t me1 = nullptr)
 {
     DPParams dp = make_params(r, alg, kind, sg_flags, open, gap, check);
     const bool wave = kind == PHIP_SCORE && !r->mx.is_pssm && !(check && alg != PHIP_SW);
@@ -361,7 +374,7 @@ static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int
             r->plans.push_back(plan);
         }
         HIP_OK(hipEventRecord(g_ev0, g_stream));
-        if (wave_plan_run(plan, dp, g_stream) != 0) return -1;
+        if (wave_plan_run(plan, dp, g_stream, me0, me1) != 0) return -1;
     } else {
         // the int32 wavefront kernel for the pairs it covers, the general
         // tier for the rest
@@ -854,6 +867,97 @@ extern "C" int phip_resident_align4(void *handle, int alg, int sg_flags, int ope
     } while (0);
     pthread_mutex_unlock(&g_lock);
     return rc;
+}
+
+// Several score passes over one resident batch in one call: every pass is
+// launched (kernels, packing, download into its own pinned buffer) before
+// the host waits for the first, so the host's work on a pass's results
+// overlaps the next passes' kernels.  phip_resident_launch4 takes the device
+// lock and phip_resident_end4 releases it; in between the caller collects
+// each pass with phip_resident_wait4.
+extern "C" int phip_resident_launch4(void *handle, int npass, const int *alg, const int *sg_flags,
+                                     int open, int gap)
+{
+    phip_resident_t *r = (phip_resident_t *)handle;
+    if (!r || npass < 1 || npass > phip_resident_t::kPasses || !alg || !sg_flags) {
+        phip_set_error("phip_resident_launch4: bad arguments");
+        return -1;
+    }
+    pthread_mutex_lock(&g_lock);
+    int rc = 0;
+    for (int k = 0; k < npass && rc == 0; ++k) {
+        if (!r->pd4[k]) r->pd4[k] = (phip_out4_t *)dev_alloc((size_t)r->np * sizeof(phip_out4_t));
+        if (!r->ph4[k] && hipHostMalloc((void **)&r->ph4[k], (size_t)r->np * sizeof(phip_out4_t),
+                                        hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            r->ph4[k] = nullptr;
+        }
+        for (int e = 0; e < 5 && rc == 0; ++e)
+            if (!r->pev[k][e] && hipEventCreate(&r->pev[k][e]) != hipSuccess) rc = -1;
+        if (rc || !r->pd4[k] || !r->ph4[k]) {
+            phip_set_error("phip_resident_launch4: out of memory");
+            rc = -1;
+            break;
+        }
+        hipEvent_t *ev = r->pev[k];
+        if (hipEventRecord(ev[0], g_stream) != hipSuccess) { rc = -1; break; }
+        // the main-kernel events stay unrecorded unless the wavefront runs
+        rc = resident_run(r, alg[k], PHIP_SCORE, sg_flags[k], open, gap, 0, ev[2], ev[3]);
+        if (rc) break;
+        r->pass_wave[k] = r->last_wave;
+        hipLaunchKernelGGL(k_pack_out4, dim3((r->np + 255) / 256), dim3(256), 0, g_stream, r->dout,
+                           r->pd4[k], r->np);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(ev[1], g_stream) != hipSuccess ||
+            hipMemcpyAsync(r->ph4[k], r->pd4[k], (size_t)r->np * sizeof(phip_out4_t),
+                           hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+            hipEventRecord(ev[4], g_stream) != hipSuccess) {
+            phip_set_error("phip_resident_launch4: launch failed");
+            rc = -1;
+        }
+    }
+    if (rc) {
+        (void)hipStreamSynchronize(g_stream);
+        pthread_mutex_unlock(&g_lock);
+        return -1;
+    }
+    r->npass = npass;
+    return 0;
+}
+
+extern "C" int phip_resident_wait4(void *handle, int k, const phip_out4_t **res, double *kernel_ms,
+                                   double *main_ms)
+{
+    phip_resident_t *r = (phip_resident_t *)handle;
+    if (!r || k < 0 || k >= r->npass || !res) {
+        phip_set_error("phip_resident_wait4: bad arguments");
+        return -1;
+    }
+    if (hipEventSynchronize(r->pev[k][4]) != hipSuccess) {
+        phip_set_error("phip_resident_wait4: download failed");
+        return -1;
+    }
+    for (int i = 0; i < r->np; ++i)
+        if (r->ph4[k][i].status & PHIP_ST_ERR) {
+            phip_set_error("pair %d could not be computed (long-pair pipeline failed)", i);
+            return -1;
+        }
+    float ms = 0.f, mm = 0.f;
+    if (hipEventElapsedTime(&ms, r->pev[k][0], r->pev[k][1]) != hipSuccess) ms = 0.f;
+    if (!r->pass_wave[k] || hipEventElapsedTime(&mm, r->pev[k][2], r->pev[k][3]) != hipSuccess) mm = ms;
+    if (kernel_ms) *kernel_ms = ms;
+    if (main_ms) *main_ms = mm;
+    g_last_ms = ms;
+    g_last_main_ms = mm;
+    *res = r->ph4[k];
+    return 0;
+}
+
+extern "C" void phip_resident_end4(void *handle)
+{
+    phip_resident_t *r = (phip_resident_t *)handle;
+    (void)hipStreamSynchronize(g_stream);
+    if (r) r->npass = 0;
+    pthread_mutex_unlock(&g_lock);
 }
 
 extern "C" const phip_out_t *phip_resident_results(void *handle)

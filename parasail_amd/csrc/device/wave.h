@@ -26,7 +26,9 @@ struct WavePlan;
 // not provably fit int16 are listed for the int32 tier.  NULL on failure.
 WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t s);
 // Launches the plan asynchronously on s; results land in dp.out.
-int wave_plan_run(WavePlan *p, const DPParams &dp, hipStream_t s);
+// e0/e1 (optional): also recorded around the main wavefront launch
+int wave_plan_run(WavePlan *p, const DPParams &dp, hipStream_t s, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
 void wave_plan_free(WavePlan *p);
 bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap);
 // Device time (ms, HIP events on the launch stream) of the wavefront kernel

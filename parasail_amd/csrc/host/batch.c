@@ -411,6 +411,61 @@ PHOST_EXPORT int parasail_hip_batch_align(parasail_hip_batch_t *hb, const char *
     return 0;
 }
 
+/* per pass of the last parasail_hip_batch_align_multi: all kernels, main kernel */
+static double g_pass_ms[4][2];
+static int g_npass_ms = 0;
+
+PHOST_EXPORT int parasail_hip_batch_align_multi(parasail_hip_batch_t *hb, int nalg,
+                                                const char *const *alg_names, int open, int gap,
+                                                parasail_hip_batch_result_t *out)
+{
+    int alg[4], flags[4];
+    if (!hb || nalg < 1 || nalg > 4 || !alg_names || !out || open < 0 || gap < 0) {
+        fprintf(stderr, "%s: invalid arguments\n", __func__);
+        return -1;
+    }
+    for (int k = 0; k < nalg; ++k)
+        if (!alg_names[k] || parse_alg(alg_names[k], &alg[k], &flags[k]) != 0) {
+            fprintf(stderr, "%s: invalid arguments\n", __func__);
+            return -1;
+        }
+    if (phip_resident_launch4(hb->dev, nalg, alg, flags, open, gap) != 0) {
+        fprintf(stderr, "%s: %s\n", __func__, phip_last_error());
+        return -1;
+    }
+    int rc = 0;
+    g_npass_ms = 0;
+    for (int k = 0; k < nalg && rc == 0; ++k) {
+        const phip_out4_t *res = NULL;
+        if (phip_resident_wait4(hb->dev, k, &res, &g_pass_ms[k][0], &g_pass_ms[k][1]) != 0) {
+            fprintf(stderr, "%s: %s\n", __func__, phip_last_error());
+            rc = -1;
+            break;
+        }
+        g_npass_ms = k + 1;
+        int f0 = alg[k] == PHIP_SW ? PARASAIL_FLAG_SW : alg[k] == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
+        if (alg[k] == PHIP_SG) {
+            if (flags[k] & PHIP_S1_BEG) f0 |= PARASAIL_FLAG_SG_S1_BEG;
+            if (flags[k] & PHIP_S1_END) f0 |= PARASAIL_FLAG_SG_S1_END;
+            if (flags[k] & PHIP_S2_BEG) f0 |= PARASAIL_FLAG_SG_S2_BEG;
+            if (flags[k] & PHIP_S2_END) f0 |= PARASAIL_FLAG_SG_S2_END;
+        }
+        f0 |= PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_LANES_64;
+        unpack_ctx_t c = {res, &out[k], hb->matrix, alg[k], open, f0};
+        phost_parallel_for(hb->npairs, 50000, unpack_range, &c);
+    }
+    phip_resident_end4(hb->dev);
+    return rc;
+}
+
+PHOST_EXPORT int parasail_hip_last_pass_ms(int k, double *kernel_ms, double *main_kernel_ms)
+{
+    if (k < 0 || k >= g_npass_ms) return -1;
+    if (kernel_ms) *kernel_ms = g_pass_ms[k][0];
+    if (main_kernel_ms) *main_kernel_ms = g_pass_ms[k][1];
+    return 0;
+}
+
 PHOST_EXPORT void parasail_hip_batch_free(parasail_hip_batch_t *hb)
 {
     if (!hb) return;

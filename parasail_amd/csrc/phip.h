@@ -111,6 +111,15 @@ typedef struct phip_out4 {
  * the handle's pinned buffer (*res, valid until the next call or free). */
 int phip_resident_align4(void *handle, int alg, int sg_flags, int open, int gap,
                          const phip_out4_t **res);
+/* Several score passes over a resident batch in one call (at most 4): launch
+ * them all (this takes the device lock), collect pass k's packed results with
+ * phip_resident_wait4 (its kernel time, and its main kernel's, in ms), then
+ * release the lock with phip_resident_end4, also after a failed wait. */
+int phip_resident_launch4(void *handle, int npass, const int *alg, const int *sg_flags, int open,
+                          int gap);
+int phip_resident_wait4(void *handle, int k, const phip_out4_t **res, double *kernel_ms,
+                        double *main_ms);
+void phip_resident_end4(void *handle);
 /* out == NULL: into the handle's pinned buffer, phip_resident_results() */
 int phip_resident_align(void *handle, int alg, int sg_flags, int open, int gap, phip_out_t *out);
 const phip_out_t *phip_resident_results(void *handle);

@@ -5,6 +5,7 @@ and CIGAR, for SW, NW and the eleven SG variants, per call and batched,
 square matrices and PSSMs, plus the reference's known answers
 (tests/test_basic.c:20-85).  The oracle is pinned in tests/test_oracle.py.
 """
+import ctypes
 import random
 
 import numpy as np
@@ -513,3 +514,89 @@ def test_long_nw_int32_table_scores(mats):
     got = P.align_batch("nw", 0, 32, [qd], [td], 20, 2, dp_)
     exp, _ = O.batch(O.NW, O.SCORE, [do.map(qd)], [do.map(td)], 20, 2, do)
     assert got["score"][0] == exp[0, 0]
+
+
+def _resident_api():
+    L = P.lib()
+    I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
+    L.parasail_hip_batch_create.restype = ctypes.c_void_p
+    L.parasail_hip_batch_create.argtypes = [ctypes.c_char_p, I64, I32, ctypes.c_char_p, I64, I32,
+                                            ctypes.c_int, P.MP]
+    L.parasail_hip_batch_align.restype = ctypes.c_int
+    L.parasail_hip_batch_align.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
+                                           ctypes.c_int, ctypes.POINTER(P.BatchResult)]
+    L.parasail_hip_batch_align_multi.restype = ctypes.c_int
+    L.parasail_hip_batch_align_multi.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
+                                                 ctypes.c_int, ctypes.POINTER(P.BatchResult)]
+    L.parasail_hip_last_pass_ms.restype = ctypes.c_int
+    L.parasail_hip_last_pass_ms.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]
+    L.parasail_hip_batch_free.argtypes = [ctypes.c_void_p]
+    return L
+
+
+def _bufs(n):
+    arrs = {k: np.zeros(n, dtype=np.int32) for k in ("score", "end_query", "end_ref", "flag")}
+    br = P.BatchResult()
+    for k, v in arrs.items():
+        setattr(br, k, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return arrs, br
+
+
+@pytest.mark.gpu
+def test_resident_batch_single_and_multi(mats):
+    """The HBM-resident batch (parasail_hip_batch_create/align) returns what
+    the per-call batch entry point returns (itself checked against the
+    oracle above), and parasail_hip_batch_align_multi returns, pass by pass,
+    what the single-algorithm calls return, whatever the order of its
+    algorithms and on repeated calls."""
+    from parasail_amd import workloads as WL
+    pm, om = mats["blosum62"]
+    b = WL.config1(seed=21, npairs=3000)
+    n = len(b)
+    L = _resident_api()
+    I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
+    qoff = np.ascontiguousarray(b.qoff, dtype=np.int64)
+    toff = np.ascontiguousarray(b.toff, dtype=np.int64)
+    qlen = np.ascontiguousarray(b.qlen, dtype=np.int32)
+    tlen = np.ascontiguousarray(b.tlen, dtype=np.int32)
+    h = L.parasail_hip_batch_create(b.q, qoff.ctypes.data_as(I64), qlen.ctypes.data_as(I32), b.t,
+                                    toff.ctypes.data_as(I64), tlen.ctypes.data_as(I32), n, pm)
+    assert h
+    try:
+        single = {}
+        for name in ("sw", "nw", "sg", "sg_qb_de"):
+            arrs, br = _bufs(n)
+            assert L.parasail_hip_batch_align(h, name.encode(), 11, 1, ctypes.byref(br)) == 0
+            ref = P.align_arrays(name, 0, 0, b, 11, 1, pm)
+            for key in ("score", "end_query", "end_ref", "flag"):
+                assert (arrs[key] == ref[key]).all(), (name, key)
+            single[name] = arrs
+        # spot check against the oracle
+        for i in range(0, n, 300):
+            q = bytes(b.q[b.qoff[i]:b.qoff[i] + b.qlen[i]])
+            t = bytes(b.t[b.toff[i]:b.toff[i] + b.tlen[i]])
+            for name in ("sw", "nw", "sg"):
+                alg, fl = oracle_alg(name)
+                exp, _ = O.align(alg, O.SCORE, q, t, 11, 1, om, fl)
+                assert (single[name]["score"][i], single[name]["end_query"][i],
+                        single[name]["end_ref"][i]) == tuple(exp[:3]), (name, i)
+        for order in (("sw", "nw", "sg"), ("sg_qb_de", "sw"), ("nw", "sg", "sw", "sg_qb_de")):
+            outs = (P.BatchResult * len(order))()
+            arrs = []
+            for k in range(len(order)):
+                a, br = _bufs(n)
+                arrs.append(a)
+                outs[k] = br
+            names = (ctypes.c_char_p * len(order))(*[x.encode() for x in order])
+            assert L.parasail_hip_batch_align_multi(h, len(order), names, 11, 1, outs) == 0
+            for k, name in enumerate(order):
+                for key in ("score", "end_query", "end_ref", "flag"):
+                    assert (arrs[k][key] == single[name][key]).all(), (order, name, key)
+                km, mm = ctypes.c_double(), ctypes.c_double()
+                assert L.parasail_hip_last_pass_ms(k, ctypes.byref(km), ctypes.byref(mm)) == 0
+                assert km.value > 0 and 0 < mm.value <= km.value + 1e-3
+            assert L.parasail_hip_last_pass_ms(len(order), None, None) == -1
+    finally:
+        L.parasail_hip_batch_free(h)
