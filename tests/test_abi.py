@@ -91,3 +91,27 @@ def test_no_device_returns_null():
     assert r is None
     with pytest.raises(RuntimeError):
         P.align_batch("sw", 0, 16, ["ASDF"], ["ASDF"], 10, 1, P.blosum62())
+
+
+def test_multi_pass_rejects_bad_arguments_without_a_device():
+    """parasail_hip_batch_align_multi checks its arguments before it touches
+    the device, and a pass index with no timing reports -1."""
+    L = P.lib()
+    L.parasail_hip_batch_align_multi.restype = ctypes.c_int
+    L.parasail_hip_batch_align_multi.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
+                                                 ctypes.c_int, ctypes.POINTER(P.BatchResult)]
+    L.parasail_hip_last_pass_ms.restype = ctypes.c_int
+    L.parasail_hip_last_pass_ms.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]
+    names = (ctypes.c_char_p * 2)(b"sw", b"nw")
+    outs = (P.BatchResult * 2)()
+    assert L.parasail_hip_batch_align_multi(None, 2, names, 11, 1, outs) == -1
+    fake = ctypes.c_void_p(1)  # never dereferenced: the arguments fail first
+    assert L.parasail_hip_batch_align_multi(fake, 5, names, 11, 1, outs) == -1
+    assert L.parasail_hip_batch_align_multi(fake, 0, names, 11, 1, outs) == -1
+    assert L.parasail_hip_batch_align_multi(fake, 2, names, -1, 1, outs) == -1
+    bad = (ctypes.c_char_p * 2)(b"sw", b"xx")
+    assert L.parasail_hip_batch_align_multi(fake, 2, bad, 11, 1, outs) == -1
+    assert L.parasail_hip_last_pass_ms(-1, None, None) == -1
+    assert L.parasail_hip_last_pass_ms(7, None, None) == -1
