@@ -63,8 +63,14 @@ static void convert_range(void *ctx, int lo, int hi)
         const phip_out_t *o = &x->res[k];
         int w = width, sat = 0;
         if (w == 0) {
+            /* SW: the narrowest width whose window holds the exact score
+             * (satcheck.c tries 8, then 16, then 32); NW/SG: 16 unless an
+             * int32 tier ran, since their 8-bit window needs the table's
+             * extrema, which the batch kernels do not compute */
             w = (o->status & PHIP_ST_W32) ? 32 : 16;
-            if (alg == PHIP_SW && w == 32 && !phost_saturates(alg, 16, open, matrix, o)) w = 16;
+            if (alg == PHIP_SW)
+                w = !phost_saturates(alg, 8, open, matrix, o) ? 8
+                    : !phost_saturates(alg, 16, open, matrix, o) ? 16 : 32;
         } else {
             sat = phost_saturates(alg, w, open, matrix, o);
         }
@@ -369,16 +375,19 @@ static void unpack_range(void *ctx, int lo, int hi)
     for (int k = lo; k < hi; ++k) {
         const phip_out4_t *o = &c->res[k];
         int w = (o->status & PHIP_ST_W32) ? 32 : 16;
-        if (c->alg == PHIP_SW && w == 32) {
+        if (c->alg == PHIP_SW) {
             phip_out_t full;
             memset(&full, 0, sizeof(full));
             full.score = o->score; /* the SW width test reads the score only */
-            if (!phost_saturates(c->alg, 16, c->open, c->matrix, &full)) w = 16;
+            w = !phost_saturates(c->alg, 8, c->open, c->matrix, &full) ? 8
+                : !phost_saturates(c->alg, 16, c->open, c->matrix, &full) ? 16 : 32;
         }
         if (score) score[k] = o->score;
         if (eq) eq[k] = o->end_query;
         if (er) er[k] = o->end_ref;
-        if (fl) fl[k] = c->f0 | (w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32);
+        if (fl)
+            fl[k] = c->f0 | (w == 8 ? PARASAIL_FLAG_BITS_8
+                             : w == 16 ? PARASAIL_FLAG_BITS_16 : PARASAIL_FLAG_BITS_32);
     }
 }
 

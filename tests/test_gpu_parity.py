@@ -149,6 +149,26 @@ def test_sat_reports_narrowest_width(mats):
     assert not r["flag"] & P.FLAG_SATURATED
 
 
+def test_batch_sat_sw_reports_narrowest_width(mats):
+    """A `_sat` SW batch reports, pair by pair, the width the per-call
+    `_sat` function reports (src/satcheck.c:41520-41537 tries 8, then 16):
+    the 8-bit window holds W^10 (score 110) and random protein pairs, not
+    W^30 (score 330)."""
+    pm, _ = mats["blosum62"]
+    prng = random.Random(31)
+    qs = ["W" * 10, "W" * 30, "W" * 25] + [rseq(prng, prng.randint(20, 400)) for _ in range(40)]
+    ts = ["W" * 10, "W" * 30, "W" * 23] + [rseq(prng, prng.randint(20, 400)) for _ in range(40)]
+    got = P.align_batch("sw", 0, 0, qs, ts, 11, 1, pm)
+    bits = P.FLAG_BITS_8 | P.FLAG_BITS_16 | P.FLAG_BITS_32
+    widths = set()
+    for k, (q, t) in enumerate(zip(qs, ts)):
+        r = P.align("parasail_sw_striped_hip_64_sat", q, t, 11, 1, pm)
+        assert got["score"][k] == r["score"], k
+        assert int(got["flag"][k]) & bits == r["flag"] & bits, (k, r["score"])
+        widths.add(r["flag"] & bits)
+    assert widths == {P.FLAG_BITS_8, P.FLAG_BITS_16}
+
+
 def test_profile_functions(mats):
     L = P.lib()
     pm, om = mats["blosum62"]
