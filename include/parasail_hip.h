@@ -462,6 +462,35 @@ void parasail_hip_server_free(parasail_hip_server_t *srv);
 /* monotonic clock (seconds) of the serving loop */
 double parasail_hip_now(void);
 
+/* FASTA/FASTQ ingest (parasail/io.h:33-59; src/parser.c:48-259): the
+ * records of a file, and their residues packed back to back with a '$'
+ * after each, ready for parasail_hip_batch_create.  Same struct layouts. */
+typedef struct parasail_string {
+    size_t l;
+    char *s;
+} parasail_string_t;
+typedef struct parasail_sequence {
+    parasail_string_t name;
+    parasail_string_t comment;
+    parasail_string_t seq;
+    parasail_string_t qual;
+} parasail_sequence_t;
+typedef struct parasail_sequences {
+    parasail_sequence_t *seqs;
+    size_t l;
+    size_t characters;
+    size_t shortest;
+    size_t longest;
+    float mean;
+    float stddev;
+} parasail_sequences_t;
+/* replaces parasail_sequences_from_file (parasail/io.h:55) */
+parasail_sequences_t *parasail_sequences_from_file(const char *fname);
+/* replaces parasail_sequences_pack (parasail/io.h:57) */
+char *parasail_sequences_pack(const parasail_sequences_t *sequences, size_t *size);
+/* replaces parasail_sequences_free (parasail/io.h:59) */
+void parasail_sequences_free(parasail_sequences_t *sequences);
+
 /* device time (ms, HIP events) of the last call's alignment kernels */
 double parasail_hip_last_kernel_ms(void);
 /* device time (ms, HIP events on its stream) of the last call's dominant

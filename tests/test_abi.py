@@ -115,3 +115,71 @@ def test_multi_pass_rejects_bad_arguments_without_a_device():
     assert L.parasail_hip_batch_align_multi(fake, 2, bad, 11, 1, outs) == -1
     assert L.parasail_hip_last_pass_ms(-1, None, None) == -1
     assert L.parasail_hip_last_pass_ms(7, None, None) == -1
+
+
+def test_sequences_from_file_and_pack(tmp_path):
+    """FASTA and FASTQ records as src/parser.c:48-221 reads them through
+    contrib/kseq.h:177-218: name up to the first whitespace, the rest of the
+    header line as comment, wrapped and CRLF lines joined, empty lines
+    skipped, FASTQ quality read to the sequence's length; length statistics
+    of parasail/stats.h; packing of src/parser.c:235-259."""
+    import ctypes
+    from parasail_amd import lib
+
+    class Str(ctypes.Structure):
+        _fields_ = [("l", ctypes.c_size_t), ("s", ctypes.c_char_p)]
+
+    class Seq(ctypes.Structure):
+        _fields_ = [("name", Str), ("comment", Str), ("seq", Str), ("qual", Str)]
+
+    class Seqs(ctypes.Structure):
+        _fields_ = [("seqs", ctypes.POINTER(Seq)), ("l", ctypes.c_size_t),
+                    ("characters", ctypes.c_size_t), ("shortest", ctypes.c_size_t),
+                    ("longest", ctypes.c_size_t), ("mean", ctypes.c_float),
+                    ("stddev", ctypes.c_float)]
+
+    L = lib()
+    L.parasail_sequences_from_file.restype = ctypes.POINTER(Seqs)
+    L.parasail_sequences_from_file.argtypes = [ctypes.c_char_p]
+    L.parasail_sequences_pack.restype = ctypes.c_void_p
+    L.parasail_sequences_pack.argtypes = [ctypes.POINTER(Seqs), ctypes.POINTER(ctypes.c_size_t)]
+    L.parasail_sequences_free.argtypes = [ctypes.POINTER(Seqs)]
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+
+    def read(text):
+        f = tmp_path / "in.txt"
+        f.write_bytes(text)
+        p = L.parasail_sequences_from_file(str(f).encode())
+        assert p
+        s = p.contents
+        recs = [tuple(getattr(s.seqs[i], k).s or b"" for k in ("name", "comment", "seq", "qual"))
+                for i in range(s.l)]
+        for i in range(s.l):
+            for k in ("name", "comment", "seq", "qual"):
+                x = getattr(s.seqs[i], k)
+                assert x.l == len(x.s or b"")
+        stats = (s.characters, s.shortest, s.longest, s.mean, s.stddev)
+        n = ctypes.c_size_t()
+        packed = L.parasail_sequences_pack(p, ctypes.byref(n))
+        pk = ctypes.string_at(packed, n.value + 1)
+        libc.free(packed)
+        L.parasail_sequences_free(p)
+        return recs, stats, pk
+
+    recs, stats, pk = read(b"junk line\n>sp|P1 first protein\nMKV\nLLA\n\n>P2\r\nAC\r\nD\r\n"
+                           b">P3\tx y \n" + b"W" * 9 + b"\n")
+    assert recs == [(b"sp|P1", b"first protein", b"MKVLLA", b""), (b"P2", b"", b"ACD", b""),
+                    (b"P3", b"x y ", b"W" * 9, b"")]
+    assert stats[:3] == (18, 3, 9)
+    assert abs(stats[3] - 6.0) < 1e-6 and abs(stats[4] - 3.0) < 1e-6  # sample stddev of 6, 3, 9
+    assert pk == b"MKVLLA$ACD$WWWWWWWWW$\0"
+    # FASTQ: a quality line may start with '@'; wrapped quality; second record
+    recs, stats, pk = read(b"@r1 c\nACGT\nAC\n+r1\n@@II\nII\n@r2\nGG\n+\n##\n")
+    assert recs == [(b"r1", b"c", b"ACGTAC", b"@@IIII"), (b"r2", b"", b"GG", b"##")]
+    assert stats[:3] == (8, 2, 6) and pk == b"ACGTAC$GG$\0"
+    # a quality shorter than its sequence ends the parse after the records before it
+    recs, _, _ = read(b"@a\nAC\n+\nII\n@b\nACGT\n+\nI\n")
+    assert recs == [(b"a", b"", b"AC", b"II")]
+    L.parasail_sequences_free(None)
+    assert not L.parasail_sequences_from_file(str(tmp_path / "missing.fa").encode())
