@@ -5,8 +5,14 @@ The reference's sources cannot be compiled here, so the oracle is pinned by:
     tests/golden/fixtures.json);
   * brute-force enumeration of every alignment path on tiny inputs, which
     fixes scores and, through closed-form argmin rules, the end positions;
+  * brute force of every cell's trace code from path maxima (H source,
+    E/F open against extend, the SW zero mask) for SW, NW and all 16 SG
+    end-flag sets;
+  * brute force of banded NW over the paths that stay inside the band of
+    nw_banded.c:92-97, boundary moves included;
   * invariants: a CIGAR re-scored with the matrix and gaps gives the score,
-    SW stats equal the counts along the CIGAR, PSSM from
+    the _stats counts of SW, NW and every SG flag set equal the counts along
+    the trace path into the stats end cell, PSSM from
     convert_square_to_pssm equals the square matrix, banded NW with a band
     covering the whole matrix equals NW, SG >= NW.
 Each check is chosen so that a plausible slip (wrong boundary, swapped E/F,
@@ -422,3 +428,230 @@ def test_striped_unwind_roundtrip():
     c1 = O.cigar(s1, s2, False, res[1], res[2], tr)
     c2 = O.cigar(s1, s2, False, res[1], res[2], st8, striped=True, bits=16, lanes=lanes)
     assert O.cigar_string(c1[0]) == O.cigar_string(c2[0]) and c1[1:] == c2[1:]
+
+
+# ------------------------------------------------- trace codes and stats
+#
+# Every DP cell's trace code is fixed by path maxima: H is the best path
+# into the cell, H_dag the best one whose last move is diagonal, E (F) the
+# best one whose last move consumes s2 (s1), and E opens (DIAG_E) exactly
+# when the best such path whose move before the last one is not also an
+# s2 move beats every path that extends an s2 run (sg_trace.c:144-150,
+# sw_trace.c:145-147).  Enumerating paths restates none of the recurrence
+# (no max over neighbours), so a swapped E/F, a wrong open/extend tie or a
+# wrong H-source priority fails here.  Needs gap <= open: reopening a gap
+# right after one is then never better than extending it, and a path of
+# moves has one score.
+
+def cell_path_maxima(alg, q, t, M, open_, gap, flags=(0, 0, 0, 0)):
+    """For every 1-based DP cell: best score over paths into it, grouped by
+    (last move, whether the move before it is the same gap move)."""
+    m, n = len(q), len(t)
+    if alg == O.SW:
+        starts = [(a, b) for a in range(m + 1) for b in range(n + 1)]
+    else:
+        starts = [(0, 0)]
+        if alg == O.SG and flags[0]:
+            starts += [(a, 0) for a in range(1, m + 1)]
+        if alg == O.SG and flags[2]:
+            starts += [(0, b) for b in range(1, n + 1)]
+    best = {}
+
+    def note(cell, key, s):
+        d = best.setdefault(cell, {})
+        if key not in d or s > d[key]:
+            d[key] = s
+
+    def rec(i, j, s, prev, prev2):
+        if i >= 1 and j >= 1:
+            note((i, j), (prev, prev2 == prev), s)
+        if i < m and j < n:
+            rec(i + 1, j + 1, s + M[q[i], t[j]], "D", prev)
+        if i < m:
+            rec(i + 1, j, s - (gap if prev == "U" else open_), "U", prev)
+        if j < n:
+            rec(i, j + 1, s - (gap if prev == "L" else open_), "L", prev)
+
+    for (a, b) in starts:
+        # NW/SG: the boundary row and column are reached only by the moves
+        # that start at (0, 0) or at a free start, as the DP initialises them
+        rec(a, b, 0, None, None)
+    return best
+
+
+def expected_trace(alg, q, t, M, open_, gap, flags=(0, 0, 0, 0)):
+    m, n = len(q), len(t)
+    best = cell_path_maxima(alg, q, t, M, open_, gap, flags)
+    codes = np.zeros((m, n), dtype=np.int64)
+    ninf = -10 ** 9
+    for i in range(1, m + 1):
+        for j in range(1, n + 1):
+            d = best.get((i, j), {})
+            g = lambda k: d.get(k, ninf)  # noqa: E731
+            hdag = max(g(("D", False)), g(("D", True)))
+            f_opn, f_ext = g(("U", False)), g(("U", True))
+            e_opn, e_ext = g(("L", False)), g(("L", True))
+            F, E = max(f_opn, f_ext), max(e_opn, e_ext)
+            H = max(hdag, E, F, 0 if alg == O.SW else ninf)
+            code = (32 if f_opn > f_ext else 64) | (8 if e_opn > e_ext else 16)
+            code |= 4 if H == hdag else (2 if H == F else 1)
+            if alg == O.SW and H == 0:
+                code &= 120
+            codes[i - 1, j - 1] = code
+    return codes
+
+
+def _tiny_cases(rng, count):
+    for _ in range(count):
+        mx, alpha = small_matrix(rng)
+        m, n = rng.randint(1, 4), rng.randint(1, 4)
+        s1 = "".join(rng.choice(alpha) for _ in range(m))
+        s2 = "".join(rng.choice(alpha) for _ in range(n))
+        o, g = rng.choice([(1, 1), (3, 1), (5, 2), (2, 0), (2, 2), (4, 1)])
+        yield mx, s1, s2, o, g
+
+
+@pytest.mark.parametrize("alg", [O.SW, O.NW])
+def test_trace_table_brute_force_sw_nw(alg):
+    rng = random.Random(2000 + alg)
+    for mx, s1, s2, o, g in _tiny_cases(rng, 80):
+        q, t = mx.map(s1), mx.map(s2)
+        exp = expected_trace(alg, q, t, mx.values(), o, g)
+        _, tr = O.align(alg, O.TRACE, s1, s2, o, g, mx)
+        assert (tr.astype(np.int64) == exp).all(), (alg, s1, s2, o, g, tr, exp)
+
+
+def test_trace_table_brute_force_sg_all_flags():
+    rng = random.Random(2002)
+    for flags in SG_FLAGS:
+        for mx, s1, s2, o, g in _tiny_cases(rng, 10):
+            q, t = mx.map(s1), mx.map(s2)
+            exp = expected_trace(O.SG, q, t, mx.values(), o, g, flags)
+            _, tr = O.align(O.SG, O.TRACE, s1, s2, o, g, mx, flags)
+            assert (tr.astype(np.int64) == exp).all(), (flags, s1, s2, o, g, tr, exp)
+
+
+def counts_along_cigar(ops, bq, br, end_query, end_ref, q, t, M):
+    """(matches, similar, length) over the moves of a CIGAR that enter a DP
+    cell of the matrix (row and column >= 1), up to the end cell.  Moves
+    along the DP's boundary row or column carry no statistics: the
+    boundary's counts are 0 (sg_stats.c:146-170,189-201, nw_stats.c,
+    sw_stats.c), and end gaps past the end cell are not part of the DP."""
+    i, j = bq, br
+    mt = sm = ln = 0
+    for v in ops:
+        cnt, op = int(v) >> 4, "MIDNSHP=XB"[int(v) & 0xF]
+        for _ in range(cnt):
+            if (i, j) == (end_query + 1, end_ref + 1):
+                return mt, sm, ln
+            if op in "=X":
+                mt += int(q[i] == t[j])
+                sm += int(M[q[i], t[j]] > 0)
+                i += 1
+                j += 1
+            elif op == "I":
+                i += 1
+            else:
+                j += 1
+            if i >= 1 and j >= 1:
+                ln += 1
+    return mt, sm, ln
+
+
+def _stats_vs_path(alg, s1, s2, o, g, mx, flags):
+    st, _ = O.align(alg, O.STATS, s1, s2, o, g, mx, flags)
+    sc, tr = O.align(alg, O.TRACE, s1, s2, o, g, mx, flags)
+    # the stats functions' end rules may pick another cell than the score
+    # function's (sg_stats.c:252-309 against sg.c:171-235); the path into
+    # the stats end cell comes from the same trace table
+    c = O.cigar(s1, s2, alg == O.SG, st[1], st[2], tr)
+    assert c is not None, (alg, flags, s1, s2, st)
+    ops, bq, br = c
+    q, t = mx.map(s1), mx.map(s2)
+    exp = counts_along_cigar(ops, bq, br, st[1], st[2], q, t, mx.values())
+    assert st[3:] == exp, (alg, flags, s1, s2, o, g, st, exp, O.cigar_string(ops), bq, br)
+    if alg != O.SG:
+        assert st[:3] == sc[:3]
+
+
+@pytest.mark.parametrize("alg", [O.SW, O.NW])
+def test_stats_are_counts_along_the_trace_path(alg):
+    rng = random.Random(3000 + alg)
+    b = O.blosum62()
+    dna = O.matrix_create("ACGT", 2, -3)
+    for trial in range(150):
+        mx, alpha = (b, "ARNDCQEGHILKMFPSTWYV") if trial % 2 else (dna, "ACGT")
+        s1 = rand_seq(rng, alpha, rng.randint(1, 30))
+        s2 = rand_seq(rng, alpha, rng.randint(1, 30))
+        if trial % 3 == 0:
+            s2 = s1[rng.randint(0, len(s1) - 1):] + rand_seq(rng, alpha, rng.randint(0, 4))
+        o, g = rng.choice([(11, 1), (10, 1), (5, 2), (14, 2), (3, 3), (1, 1)])
+        _stats_vs_path(alg, s1, s2, o, g, mx, (0, 0, 0, 0))
+
+
+def test_stats_are_counts_along_the_trace_path_sg_all_flags():
+    rng = random.Random(3002)
+    b = O.blosum62()
+    dna = O.matrix_create("ACGT", 2, -3)
+    for flags in SG_FLAGS:
+        for trial in range(25):
+            mx, alpha = (b, "ARNDCQEGHILKMFPSTWYV") if trial % 2 else (dna, "ACGT")
+            s1 = rand_seq(rng, alpha, rng.randint(1, 30))
+            s2 = rand_seq(rng, alpha, rng.randint(1, 30))
+            if trial % 3 == 0:
+                s2 = s1[rng.randint(0, len(s1) - 1):] + rand_seq(rng, alpha, rng.randint(0, 4))
+            o, g = rng.choice([(11, 1), (10, 1), (5, 2), (14, 2), (3, 3), (1, 1)])
+            _stats_vs_path(O.SG, s1, s2, o, g, mx, flags)
+
+
+# ------------------------------------------------------ banded brute force
+#
+# The band of nw_banded.c:92-97 in diagonal terms: cell (r, j) (0-based,
+# r = -1 / j = -1 the DP's boundary) lies in it when
+#   min(0, m - n) - k <= r - j <= max(0, m - n) + k,
+# i.e. the start and end diagonals with k diagonals of slack.  Every path
+# from (-1, -1) to (m-1, n-1) that stays inside the band, boundary moves
+# included (the top row is set only while row -1 is in a column's band,
+# nw_banded.c:134-136; the left column's initial slots, :118-124), competes;
+# the banded score is the best of them.
+
+def banded_brute(q, t, M, open_, gap, k):
+    m, n = len(q), len(t)
+    lo, hi = min(0, m - n) - k, max(0, m - n) + k
+    best = [None]
+
+    def inband(r, j):
+        return lo <= r - j <= hi
+
+    def rec(r, j, s, prev):
+        if (r, j) == (m - 1, n - 1):
+            if best[0] is None or s > best[0]:
+                best[0] = s
+            return
+        if r + 1 < m and j + 1 < n and inband(r + 1, j + 1):
+            rec(r + 1, j + 1, s + M[q[r + 1], t[j + 1]], "D")
+        if r + 1 < m and inband(r + 1, j):
+            rec(r + 1, j, s - (gap if prev == "U" else open_), "U")
+        if j + 1 < n and inband(r, j + 1):
+            rec(r, j + 1, s - (gap if prev == "L" else open_), "L")
+
+    rec(-1, -1, 0, None)
+    return best[0]
+
+
+def test_banded_brute_force():
+    rng = random.Random(12)
+    for trial in range(400):
+        mx, alpha = small_matrix(rng)
+        m, n = rng.randint(1, 6), rng.randint(1, 6)
+        if trial % 3 == 0:
+            n = m
+        s1 = "".join(rng.choice(alpha) for _ in range(m))
+        s2 = "".join(rng.choice(alpha) for _ in range(n))
+        o, g = rng.choice([(1, 1), (3, 1), (5, 2), (2, 0), (4, 1), (2, 2)])
+        k = rng.choice([1, 1, 2, 3, 4])
+        exp = banded_brute(mx.map(s1), mx.map(s2), mx.values(), o, g, k)
+        got = O.nw_banded(s1, s2, o, g, k, mx)
+        assert exp is not None
+        assert got[0] == exp, (s1, s2, o, g, k, got, exp)
+        assert got[1:3] == (m - 1, n - 1)
