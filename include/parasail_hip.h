@@ -278,9 +278,10 @@ int parasail_result_get_similar(const parasail_result_t *const result);
 int parasail_result_get_length(const parasail_result_t *const result);
 int *parasail_result_get_trace_table(const parasail_result_t *const result);
 
-/* striped trace table -> row-major (src/striped_unwind.c:5-26,
- * parasail/memory.h:137-141) */
-int *parasail_striped_unwind(int lena, int lenb, parasail_result_t *result);
+/* striped or scan table -> row-major int table (src/striped_unwind.c:
+ * 5-26, :56-111; parasail/memory.h:137-141): `array` is the result's trace
+ * table (or any table of the result's layout) */
+int *parasail_striped_unwind(int lena, int lenb, parasail_result_t *result, void *array);
 #endif /* _PARASAIL_H_ */
 
 /* ---- the hip family (drop-ins) ----
@@ -463,8 +464,12 @@ void parasail_hip_server_free(parasail_hip_server_t *srv);
 double parasail_hip_now(void);
 
 /* FASTA/FASTQ ingest (parasail/io.h:33-59; src/parser.c:48-259): the
- * records of a file, and their residues packed back to back with a '$'
- * after each, ready for parasail_hip_batch_create.  Same struct layouts. */
+ * records of a file (plain or gzip), and their residues packed back to back
+ * with a '$' after each; the residues of record i start after the i-th '$'
+ * and run to the next, which gives the offsets and lengths
+ * parasail_hip_batch_create takes.  A record may have an empty sequence, as
+ * in the reference; a batch rejects empty sequences, so leave those records
+ * out of it.  Same struct layouts. */
 typedef struct parasail_string {
     size_t l;
     char *s;

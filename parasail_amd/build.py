@@ -58,7 +58,7 @@ def build(verbose=False, jobs=None):
     objs = [t[-1] for t in tasks]
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
-         ["-lpthread"])
+         ["-lpthread", "-lz"])
     os.replace(tmp, LIB)
     return LIB
 

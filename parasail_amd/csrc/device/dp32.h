@@ -7,11 +7,14 @@
 #include "wave.h"
 
 // Whether a pair of this length and kind goes to the wavefront kernel.
-// check: 0, or the width whose saturation the caller tests; the kernel
-// returns no extrema, so it takes 16 only where the 16-bit window provably
-// holds every value.
+// check: 0, or the width whose saturation the caller tests (the kernel then
+// returns the pair's exact extrema).
 bool dp32_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap, int m,
                    int n);
+// Whether every value of a pair provably stays inside the `check`-bit
+// windows of the reference's kernels, so no extrema are needed.
+bool dp32_window_holds(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap,
+                       int m, int n);
 // Whether one pair on its own (a per-call alignment) goes to the kernel with
 // 64-lane jobs; score-only pairs go whatever the check (extrema computed).
 bool dp32_eligible_one(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap,

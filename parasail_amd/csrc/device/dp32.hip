@@ -64,9 +64,9 @@ struct Dp32Cfg {
     phip_out_t *out;
 };
 
-// KIND: PHIP_SCORE, PHIP_STATS or PHIP_TRACE.  EXT (score only): also the
-// exact extrema of H, E and F over the pair's cells and boundaries, for the
-// caller's saturation test (k_simple's `check`).
+// KIND: PHIP_SCORE, PHIP_STATS or PHIP_TRACE.  EXT: also the exact extrema of H, E and F over the pair's cells and boundaries, the
+// smallest cell H and (_stats) the largest length count of a cell's H, for
+// the caller's saturation test (k_simple's `check`).
 template <int R, int KIND, int ALG, bool EXT>
 __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, const int2 *tab,
                                          const int lane)
@@ -112,7 +112,7 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
     // end tracking: SW best (value, column, row, stats); NW/SG the cell
     // (m-1, n-1); SG the last column over rows < m-1 and the last row
     int best = kNeg, bj = 0, bi = 0, bS = 0;
-    int vmin = 0, vmax = 0;
+    int vmin = 0, vmax = 0, vminh = 0, vmaxl = 0;
     const int nvalid = min(max(m - t * R, 0), R);  // rows of this lane inside the query
     int colv = kNeg, colr = 0, colS = 0;
     int rowv = kNeg, rowj = 0, rowS = 0;
@@ -189,6 +189,8 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
                 if (EXT && r < nvalid) {
                     vmin = min(vmin, min(h, min(En, F)));
                     vmax = max(vmax, h);
+                    vminh = min(vminh, h);
+                    if (STATS) vmaxl = max(vmaxl, HS[r] & 0xfff);
                 }
                 t1 = t1n;
             }
@@ -323,6 +325,8 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
         for (int k = 1; k < T; k <<= 1) {
             vmin = min(vmin, __shfl_xor(vmin, k, T));
             vmax = max(vmax, __shfl_xor(vmax, k, T));
+            vminh = min(vminh, __shfl_xor(vminh, k, T));
+            vmaxl = max(vmaxl, __shfl_xor(vmaxl, k, T));
         }
         // the DP's boundary row and column (k_simple's first row and WH)
         if (!zero_top) vmin = min(vmin, -open - (nreal - 1) * gap);
@@ -342,6 +346,8 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
         o.status = J.status;
         o.min_val = vmin;
         o.max_val = vmax;
+        o.min_h = vminh;
+        o.max_len = vmaxl;
         c.out[pr] = o;
     }
 }
@@ -566,15 +572,25 @@ bool dp32_eligible(const phip_matrix_t &mx, int alg, int kind, int check, int op
     int T, R;
     if (!shape(m, T, R)) return false;
     if (kind == PHIP_STATS && (std::min(m, n) > 1023 || m + n > 4095)) return false;
+    // with a check the kernel computes the extrema (EXT)
+    return true;
+}
+
+bool dp32_window_holds(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap,
+                       int m, int n)
+{
     if (check == 0) return true;
-    if (check != 16 || alg == PHIP_SW) return false;
+    const int64_t wmax = (1 << (check - 1)) - 1, wmin = -wmax - 1;
+    const int64_t neg = (-open < mx.min ? wmin + open : wmin - (int64_t)mx.min) + 1;
+    const int64_t pos = wmax - (int64_t)mx.max - 1;
+    // a cell's length count is at most m + n, its matches and similar no more
+    if (kind == PHIP_STATS && (int64_t)m + n > pos) return false;
+    // SW's windows on the score are tested on the exact score
+    if (alg == PHIP_SW) return true;
     // every H/E/F value lies in [lo, hi]: a cell is at least its all-gap
     // path, E/F at most `open` below an H; H at most min(m, n) best matches
     const int64_t lo = -(int64_t)4 * open - (int64_t)(m + n) * gap - std::abs(mx.min) - 64;
     const int64_t hi = (int64_t)std::min(m, n) * std::max(mx.max, 0) + mx.max + open + 64;
-    // the 16-bit saturation window of the host's predicate
-    const int64_t neg = (-open < mx.min ? -32768 + (int64_t)open : -32768 - (int64_t)mx.min) + 1;
-    const int64_t pos = 32767 - (int64_t)mx.max - 1;
     return lo >= neg && hi <= pos;
 }
 
@@ -714,7 +730,7 @@ int dp32_launch(const DPParams &dp, const Dp32Plan &P, const char *region, hipSt
     const size_t lds = (size_t)(dp.msize + 1) * dp.msize * 8;
     const dim3 grid((P.nwaves + 3) / 4), block(256);
     typedef void (*Fn)(Dp32Cfg);
-    static const Fn fns[4][3] = {
+    static const Fn fns[6][3] = {
         {k_dp32<PHIP_SCORE, PHIP_SW, false>, k_dp32<PHIP_SCORE, PHIP_NW, false>,
          k_dp32<PHIP_SCORE, PHIP_SG, false>},
         {k_dp32<PHIP_SCORE, PHIP_SW, true>, k_dp32<PHIP_SCORE, PHIP_NW, true>,
@@ -722,9 +738,14 @@ int dp32_launch(const DPParams &dp, const Dp32Plan &P, const char *region, hipSt
         {k_dp32<PHIP_STATS, PHIP_SW, false>, k_dp32<PHIP_STATS, PHIP_NW, false>,
          k_dp32<PHIP_STATS, PHIP_SG, false>},
         {k_dp32<PHIP_TRACE, PHIP_SW, false>, k_dp32<PHIP_TRACE, PHIP_NW, false>,
-         k_dp32<PHIP_TRACE, PHIP_SG, false>}};
+         k_dp32<PHIP_TRACE, PHIP_SG, false>},
+        {k_dp32<PHIP_STATS, PHIP_SW, true>, k_dp32<PHIP_STATS, PHIP_NW, true>,
+         k_dp32<PHIP_STATS, PHIP_SG, true>},
+        {k_dp32<PHIP_TRACE, PHIP_SW, true>, k_dp32<PHIP_TRACE, PHIP_NW, true>,
+         k_dp32<PHIP_TRACE, PHIP_SG, true>}};
     if (dp.alg < 0 || dp.alg > 2 || dp.kind < 0 || dp.kind > 2) return -1;
-    const int row = dp.kind == PHIP_SCORE ? (dp.check ? 1 : 0) : dp.kind + 1;
+    const int row = dp.kind == PHIP_SCORE ? (dp.check ? 1 : 0)
+                    : dp.kind == PHIP_STATS ? (dp.check ? 4 : 2) : (dp.check ? 5 : 3);
     hipLaunchKernelGGL(fns[row][dp.alg], grid, block, lds, st, c);
     HIP_OK(hipGetLastError());
     return 0;

@@ -73,6 +73,7 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
     int matches = 0, similar = 0, length = 0;
     if (alg == PHIP_SG && stats) { end_query = m - 1; end_ref = n - 1; }
     int vmin = 0, vmax = 0;  // exact extrema over H/E/F (check mode)
+    int vminh = 0, vmaxl = 0;  // smallest cell H, largest cell H length
 
     // first row
     at(r.H, 0, stride, slot) = 0;
@@ -154,6 +155,8 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
             if (p.check) {
                 vmin = imin(vmin, imin(H, imin(E, F)));
                 vmax = imax(vmax, H);
+                vminh = imin(vminh, H);
+                if (stats) vmaxl = imax(vmaxl, WHL);
             }
             WH = H;
             at(r.H, j, stride, slot) = H;
@@ -235,5 +238,7 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
     o.status = PHIP_ST_W32;
     o.min_val = vmin;
     o.max_val = vmax;
+    o.min_h = vminh;
+    o.max_len = vmaxl;
     p.out[pr] = o;
 }

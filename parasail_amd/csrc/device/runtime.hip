@@ -151,6 +151,7 @@ struct phip_resident {
     // pass (0 row-major, else dp32_layout)
     Dp32Plan dplan;
     int dplan_key[4] = {-1, -1, -1, -1};
+    bool dplan_ext = false;  // some pair of the plan needs its extrema
     char *dplan_mem = nullptr;
     std::vector<int32_t> slow, tlay;
     int32_t *dslow = nullptr, *dtlay = nullptr;
@@ -391,10 +392,14 @@ static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int
             std::vector<int32_t> status(r->np, PHIP_ST_W32);
             r->slow.clear();
             r->tlay.assign(r->np, 0);
+            r->dplan_ext = false;
             for (int i = 0; i < r->np; ++i) {
                 if (dp32_eligible(r->mx, alg, kind, check, open, gap, r->qlen[i], r->tlen[i])) {
                     fast.push_back(i);
                     r->tlay[i] = dp32_layout(r->qlen[i]);
+                    if (!dp32_window_holds(r->mx, alg, kind, check, open, gap, r->qlen[i],
+                                           r->tlen[i]))
+                        r->dplan_ext = true;
                 } else {
                     r->slow.push_back(i);
                 }
@@ -434,7 +439,9 @@ static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int
             memcpy(r->dplan_key, key, sizeof(key));
         }
         HIP_OK(hipEventRecord(g_ev0, g_stream));
-        if (dp32_launch(dp, r->dplan, r->dplan_mem, g_stream) != 0) return -1;
+        DPParams dpw = dp;
+        if (!r->dplan_ext) dpw.check = 0;  // every window provably holds
+        if (dp32_launch(dpw, r->dplan, r->dplan_mem, g_stream) != 0) return -1;
         if (!r->slow.empty() && simple_run(r, dp, r->dslow, (int)r->slow.size()) != 0) return -1;
     }
     HIP_OK(hipEventRecord(g_ev1, g_stream));
@@ -620,7 +627,9 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
         dp.sg_flags = b->alg == PHIP_SG ? b->sg_flags : 0;
         dp.open = b->open;
         dp.gap = b->gap;
-        dp.check = b->kind == PHIP_SCORE ? b->want_sat_check : 0;
+        dp.check = dp32_window_holds(b->matrix, b->alg, b->kind, b->want_sat_check, b->open,
+                                     b->gap, m, n)
+                       ? 0 : b->want_sat_check;
         dp.matrix = (const int *)(d + o_mat);
         dp.msize = msize;
         dp.q = (const uint8_t *)(d + o_q);

@@ -14,10 +14,8 @@
  * 2^w-1 and ends 0, src/sw_striped_avx2_256_16.c:374-382; NW/SG: score 0,
  * ends 0, src/nw_striped_avx2_256_16.c:380-385), and "sat" reports the
  * narrowest width that holds the alignment (src/satcheck.c:41520-41537).
- * The windows are the reference's: SW saturates past 2^w - max - 2
- * (maxp = INT_MAX(w) - (max+1) on the biased scale), NW/SG when a value
- * falls below NEG_LIMIT or H exceeds POS_LIMIT
- * (src/sg_striped_avx2_256_16.c:209-210).
+ * The windows are those of the reference's kernel of the same algorithm,
+ * kind and strategy (phost_saturates below).
  */
 #include <errno.h>
 #include <limits.h>
@@ -30,16 +28,65 @@
 static int width_max(int w) { return w == 8 ? INT8_MAX : w == 16 ? INT16_MAX : INT32_MAX; }
 static int width_min(int w) { return w == 8 ? INT8_MIN : w == 16 ? INT16_MIN : INT32_MIN; }
 
-int phost_saturates(int alg, int w, int open, const parasail_matrix_t *matrix, const phip_out_t *o)
+/* Whether the reference's w-bit kernel of this algorithm, kind and
+ * strategy reports saturation, decided on the exact DP values:
+ *   SW striped score and _stats keep H biased by INT_MIN(w) and stop past
+ *   maxp = INT_MAX(w) - (max+1) on that scale, i.e. a score above
+ *   2^w - max - 2 (sw_striped_avx2_256_16.c:205-210,350-362); _stats also
+ *   when a count of a cell's H clamps at INT_MAX(w), a true 2^w - 1
+ *   (sw_stats_striped_avx2_256_16.c:440-442,612-614);
+ *   SW striped _trace is unbiased, maxp = INT_MAX(w) - (max+1)
+ *   (sw_trace_striped_avx2_256_16.c:177,403-416), and SW scan tests H (and
+ *   the _stats counts) against POS_LIMIT, the same number
+ *   (sw_scan_avx2_256_16.c:218-219,334-390, sw_stats_scan...:537-541);
+ *   NW/SG test min < NEG_LIMIT or max > POS_LIMIT
+ *   (sg_striped_avx2_256_16.c:209-210) over H, E and F (striped score and
+ *   _trace, scan _trace; sg_striped...:317-320), over H alone for the min
+ *   and H with its counts for the max (_stats, striped and scan,
+ *   sg_stats_striped...:441-445), and over H alone for nw scan score
+ *   (nw_scan_avx2_256_16.c:345-346).  sg scan score also folds in its
+ *   prefix-scan gap vector (sg_scan_avx2_256_16.c:298-304), whose length
+ *   depends on the ISA's lane count; here it takes the H/E/F window. */
+int phost_saturates(int alg, int kind, int scan, int w, int open,
+                    const parasail_matrix_t *matrix, const phip_out_t *o)
 {
     if (w >= 32) return 0;
+    const long pos = (long)width_max(w) - matrix->max - 1;
+    const long neg =
+        (-open < matrix->min ? (long)width_min(w) + open : (long)width_min(w) - matrix->min) + 1;
     if (alg == PHIP_SW) {
-        long limit = (1L << w) - matrix->max - 2;
-        return o->score > limit;
+        if (!scan && kind != PHIP_TRACE) {
+            if (o->score > (1L << w) - matrix->max - 2) return 1;
+            return kind == PHIP_STATS && o->max_len >= (1L << w) - 1;
+        }
+        if (o->score > pos) return 1;
+        return kind == PHIP_STATS && o->max_len > pos;
     }
-    long neg = (-open < matrix->min ? (long)width_min(w) + open : (long)width_min(w) - matrix->min) + 1;
-    long pos = (long)width_max(w) - matrix->max - 1;
+    if (kind == PHIP_STATS) return o->min_h < neg || o->max_val > pos || o->max_len > pos;
+    if (scan && kind == PHIP_SCORE && alg == PHIP_NW) return o->min_h < neg || o->max_val > pos;
     return o->min_val < neg || o->max_val > pos;
+}
+
+/* The values a saturated result reports: SW striped score 2^w - 1
+ * (INT_MAX(w) less the bias, sw_striped_avx2_256_16.c:378-382,412), SW
+ * striped _stats 2^(w-1) for the score and every count (0 less the bias,
+ * sw_stats_striped_avx2_256_16.c:616-623,661-667), everything else 0
+ * (sw_trace_striped...:419-423, sg_striped...:433-439); ends are 0. */
+int phost_saturated_value(int alg, int kind, int scan, int w)
+{
+    if (alg != PHIP_SW || scan || kind == PHIP_TRACE) return 0;
+    return kind == PHIP_STATS ? (int)(1L << (w - 1)) : (int)((1L << w) - 1);
+}
+
+/* The width whose saturation a call must test (phip_batch_t
+ * want_sat_check): the narrowest one its result may report. */
+int phost_check_width(int alg, int kind, int scan, int width)
+{
+    if (width == 32) return 0;
+    /* SW's windows on the score need no extrema, its _stats ones the counts */
+    if (alg == PHIP_SW && kind != PHIP_STATS) return 0;
+    (void)scan;
+    return width == 16 ? 16 : 8;
 }
 
 static int bits_flag(int w)
@@ -127,7 +174,7 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
     b.open = open;
     b.gap = gap;
     /* the narrowest width whose saturation the result must report */
-    b.want_sat_check = alg != PHIP_SW && width != 32 ? (width == 16 ? 16 : 8) : 0;
+    b.want_sat_check = phost_check_width(alg, kind, scan, width);
     b.matrix.values = matrix->matrix;
     b.matrix.size = matrix->size;
     b.matrix.length = pssm ? matrix->length : matrix->size;
@@ -161,10 +208,10 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
     int w = width;
     int saturated = 0;
     if (w == 0) {
-        w = !phost_saturates(alg, 8, open, matrix, &o) ? 8
-            : !phost_saturates(alg, 16, open, matrix, &o) ? 16 : 32;
+        w = !phost_saturates(alg, kind, scan, 8, open, matrix, &o) ? 8
+            : !phost_saturates(alg, kind, scan, 16, open, matrix, &o) ? 16 : 32;
     } else {
-        saturated = phost_saturates(alg, w, open, matrix, &o);
+        saturated = phost_saturates(alg, kind, scan, w, open, matrix, &o);
     }
 
     parasail_result_t *r = NULL;
@@ -198,14 +245,15 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
     if (kind == PHIP_STATS) r->flag |= PARASAIL_FLAG_STATS;
     if (kind == PHIP_TRACE) r->flag |= PARASAIL_FLAG_TRACE;
     if (saturated) {
+        const int sv = phost_saturated_value(alg, kind, scan, w);
         r->flag |= PARASAIL_FLAG_SATURATED;
-        r->score = alg == PHIP_SW ? (int)((1L << w) - 1) : 0;
+        r->score = sv;
         r->end_query = 0;
         r->end_ref = 0;
         if (kind == PHIP_STATS) {
-            r->stats->matches = 0;
-            r->stats->similar = 0;
-            r->stats->length = 0;
+            r->stats->matches = sv;
+            r->stats->similar = sv;
+            r->stats->length = sv;
         }
     } else {
         r->score = o.score;

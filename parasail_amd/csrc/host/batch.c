@@ -63,16 +63,19 @@ static void convert_range(void *ctx, int lo, int hi)
         const phip_out_t *o = &x->res[k];
         int w = width, sat = 0;
         if (w == 0) {
-            /* SW: the narrowest width whose window holds the exact score
-             * (satcheck.c tries 8, then 16, then 32); NW/SG: 16 unless an
-             * int32 tier ran, since their 8-bit window needs the table's
-             * extrema, which the batch kernels do not compute */
-            w = (o->status & PHIP_ST_W32) ? 32 : 16;
-            if (alg == PHIP_SW)
-                w = !phost_saturates(alg, 8, open, matrix, o) ? 8
-                    : !phost_saturates(alg, 16, open, matrix, o) ? 16 : 32;
+            /* the narrowest width whose window holds (satcheck.c tries 8,
+             * then 16, then 32), from the exact score (SW) or the exact
+             * extrema the device returned (_stats, _trace); NW/SG score-only
+             * batches report 16 unless an int32 tier ran, since their 8-bit
+             * window needs the table's extrema, which the packed-int16
+             * kernel does not compute */
+            if (alg == PHIP_SW || kind != PHIP_SCORE)
+                w = !phost_saturates(alg, kind, 0, 8, open, matrix, o) ? 8
+                    : !phost_saturates(alg, kind, 0, 16, open, matrix, o) ? 16 : 32;
+            else
+                w = (o->status & PHIP_ST_W32) ? 32 : 16;
         } else {
-            sat = phost_saturates(alg, w, open, matrix, o);
+            sat = phost_saturates(alg, kind, 0, w, open, matrix, o);
         }
         int f = alg == PHIP_SW ? PARASAIL_FLAG_SW : alg == PHIP_NW ? PARASAIL_FLAG_NW : PARASAIL_FLAG_SG;
         if (alg == PHIP_SG) {
@@ -86,12 +89,13 @@ static void convert_range(void *ctx, int lo, int hi)
         if (kind == PHIP_STATS) f |= PARASAIL_FLAG_STATS;
         if (kind == PHIP_TRACE) f |= PARASAIL_FLAG_TRACE;
         if (sat) f |= PARASAIL_FLAG_SATURATED;
-        if (out->score) out->score[k] = sat ? (alg == PHIP_SW ? (int)((1L << w) - 1) : 0) : o->score;
+        const int sv = sat ? phost_saturated_value(alg, kind, 0, w) : 0;
+        if (out->score) out->score[k] = sat ? sv : o->score;
         if (out->end_query) out->end_query[k] = sat ? 0 : o->end_query;
         if (out->end_ref) out->end_ref[k] = sat ? 0 : o->end_ref;
-        if (out->matches) out->matches[k] = sat ? 0 : o->matches;
-        if (out->similar) out->similar[k] = sat ? 0 : o->similar;
-        if (out->length) out->length[k] = sat ? 0 : o->length;
+        if (out->matches) out->matches[k] = sat ? (kind == PHIP_STATS ? sv : 0) : o->matches;
+        if (out->similar) out->similar[k] = sat ? (kind == PHIP_STATS ? sv : 0) : o->similar;
+        if (out->length) out->length[k] = sat ? (kind == PHIP_STATS ? sv : 0) : o->length;
         if (out->flag) out->flag[k] = f;
         if (x->dev_cigar) {
             const phip_cigars_t *cg = x->cg;
@@ -233,7 +237,10 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
         b.sg_flags = flags;
         b.open = open;
         b.gap = gap;
-        b.want_sat_check = alg != PHIP_SW && (width == 8 || width == 16) ? width : 0;
+        /* the extrema the flags need: none for NW/SG score-only "sat"
+         * batches (see convert_range) */
+        b.want_sat_check = width == 0 && alg != PHIP_SW && kind == PHIP_SCORE
+                               ? 0 : phost_check_width(alg, kind, 0, width);
         b.matrix.values = matrix->matrix;
         b.matrix.size = matrix->size;
         b.matrix.length = pssm ? matrix->length : matrix->size;
@@ -379,8 +386,8 @@ static void unpack_range(void *ctx, int lo, int hi)
             phip_out_t full;
             memset(&full, 0, sizeof(full));
             full.score = o->score; /* the SW width test reads the score only */
-            w = !phost_saturates(c->alg, 8, c->open, c->matrix, &full) ? 8
-                : !phost_saturates(c->alg, 16, c->open, c->matrix, &full) ? 16 : 32;
+            w = !phost_saturates(c->alg, PHIP_SCORE, 0, 8, c->open, c->matrix, &full) ? 8
+                : !phost_saturates(c->alg, PHIP_SCORE, 0, 16, c->open, c->matrix, &full) ? 16 : 32;
         }
         if (score) score[k] = o->score;
         if (eq) eq[k] = o->end_query;

@@ -44,8 +44,13 @@ void phost_encode(const phost_enc_t *e, int npairs);
 void phost_parallel_for(int n, int min_per_thread, void (*fn)(void *ctx, int lo, int hi),
                         void *ctx);
 
-int phost_saturates(int alg, int width, int open, const parasail_matrix_t *matrix,
-                    const phip_out_t *o);
+/* saturation window of the reference's w-bit kernel of (alg, kind,
+ * strategy) on exact values, the values a saturated result reports, and
+ * the width whose extrema a call must ask the device for (align.c) */
+int phost_saturates(int alg, int kind, int scan, int w, int open,
+                    const parasail_matrix_t *matrix, const phip_out_t *o);
+int phost_saturated_value(int alg, int kind, int scan, int w);
+int phost_check_width(int alg, int kind, int scan, int width);
 
 /* CIGAR walk over a row-major int8 trace (cigar.c) */
 parasail_cigar_t *phost_cigar_rowmajor(const char *seqA, int lena, const char *seqB, int lenb,

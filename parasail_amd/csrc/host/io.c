@@ -15,18 +15,23 @@
  *    returned (the reference's loop stops at kseq's -2 without an error);
  *  - the file name "stdin" (any name starting with it) reads standard input.
  * Length statistics are Welford's running mean and M2, stddev = sqrt(M2/(n-1))
- * (parasail/stats.h:36-63).  Gzip input (the reference's optional zlib path)
- * is not read.  parasail_sequences_pack follows src/parser.c:235-259.
+ * (parasail/stats.h:36-63).  The file is read through zlib, as the
+ * reference's HAVE_ZLIB build reads it (src/parser.c:50-54,86-114): gzip
+ * input is decompressed, other input is read as it is.
+ * parasail_sequences_pack follows src/parser.c:235-259.
  */
 #include <ctype.h>
 #include <math.h>
+#include <unistd.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <zlib.h>
+
 #include "internal.h"
 
-static char *read_all(FILE *fp, size_t *n)
+static char *read_all(gzFile fp, size_t *n)
 {
     size_t cap = 1 << 20, len = 0;
     char *buf = (char *)malloc(cap);
@@ -38,11 +43,12 @@ static char *read_all(FILE *fp, size_t *n)
             buf = nb;
             cap *= 2;
         }
-        size_t got = fread(buf + len, 1, cap - len, fp);
-        len += got;
+        const size_t want = cap - len > (1u << 30) ? (1u << 30) : cap - len;
+        const int got = gzread(fp, buf + len, (unsigned)want);
+        if (got < 0) { free(buf); return NULL; }
+        len += (size_t)got;
         if (got == 0) break;
     }
-    if (ferror(fp)) { free(buf); return NULL; }
     *n = len;
     return buf;
 }
@@ -97,15 +103,15 @@ PHOST_EXPORT parasail_sequences_t *parasail_sequences_from_file(const char *file
         return NULL;
     }
     const int from_stdin = strncmp("stdin", filename, 5) == 0;
-    FILE *fp = from_stdin ? stdin : fopen(filename, "rb");
+    gzFile fp = from_stdin ? gzdopen(dup(fileno(stdin)), "r") : gzopen(filename, "r");
     if (!fp) {
-        perror("fopen");
+        perror(from_stdin ? "gzdopen" : "gzopen");
         fprintf(stderr, "parasail_sequences_from_file: failed\n");
         return NULL;
     }
     size_t n = 0;
     char *b = read_all(fp, &n);
-    if (!from_stdin) fclose(fp);
+    gzclose(fp);
     parasail_sequences_t *ret = (parasail_sequences_t *)calloc(1, sizeof(*ret));
     size_t cap = 1000, count = 0, chars = 0;
     parasail_sequence_t *seqs = (parasail_sequence_t *)malloc(sizeof(*seqs) * cap);
@@ -118,6 +124,7 @@ PHOST_EXPORT parasail_sequences_t *parasail_sequences_from_file(const char *file
     while (!failed && i < n && b[i] != '>' && b[i] != '@') ++i;
     while (!failed && i < n) {
         ++i; /* the header character */
+        if (i >= n) break; /* a header character at the end: no record */
         name.l = comment.l = seq.l = qual.l = 0;
         size_t e = line_end(b, n, i), k = i;
         while (k < e && !isspace((unsigned char)b[k])) ++k;

@@ -469,10 +469,16 @@ void parasail_profile_free(parasail_profile_t *profile)
 
 /* ---------------------------------------------------------------- unwind */
 
-/* parasail_striped_unwind, src/striped_unwind.c:5-26 */
-int *parasail_striped_unwind(int lena, int lenb, parasail_result_t *result)
+/* parasail_striped_unwind (parasail/memory.h:137-141, src/striped_unwind.c:
+ * 5-26, :56-111): `array` holds a striped or scan table of lenb columns of
+ * segLen*lanes entries, entry i of column j being query row
+ * i / lanes + (i % lanes) * segLen; lanes and element width come from the
+ * result's LANES_* and BITS_* flags.  Returns the row-major int table,
+ * malloc'ed; NULL for a result that is neither striped nor scan. */
+int *parasail_striped_unwind(int lena, int lenb, parasail_result_t *result, void *array)
 {
-    if (!result || !(result->flag & PARASAIL_FLAG_TRACE)) return NULL;
+    if (!result || !array || lena <= 0 || lenb <= 0) return NULL;
+    if (!(result->flag & (PARASAIL_FLAG_STRIPED | PARASAIL_FLAG_SCAN))) return NULL;
     int lanes = phost_flag_lanes(result->flag);
     int bits = phost_flag_bits(result->flag);
     int *out = (int *)malloc(sizeof(int) * (size_t)lena * lenb);
@@ -481,7 +487,7 @@ int *parasail_striped_unwind(int lena, int lenb, parasail_result_t *result)
     for (long j = 0; j < lenb; ++j)
         for (long i = 0; i < lena; ++i) {
             long loc = j * segLen * lanes + (i % segLen) * lanes + i / segLen;
-            out[i * lenb + j] = phost_trace_at(result->trace->trace_table, bits, loc);
+            out[i * lenb + j] = phost_trace_at(array, bits, loc);
         }
     return out;
 }

@@ -62,7 +62,13 @@ typedef struct phip_out {
     int32_t score, end_query, end_ref;
     int32_t matches, similar, length;
     int32_t status;
-    int32_t min_val, max_val;  /* exact DP extrema (want_sat_check) */
+    /* exact extrema for the saturation windows (want_sat_check): the
+     * smallest H/E/F over the cells and the DP's boundary row and column,
+     * the largest H, the smallest H of a cell, and (_stats) the largest
+     * length count of a cell's H (a cell's length is never below its
+     * matches or similar counts) */
+    int32_t min_val, max_val;
+    int32_t min_h, max_len;
 } phip_out_t;
 
 /* Initialise the device context once (thread-safe).  0 on success; on a

@@ -181,5 +181,21 @@ def test_sequences_from_file_and_pack(tmp_path):
     # a quality shorter than its sequence ends the parse after the records before it
     recs, _, _ = read(b"@a\nAC\n+\nII\n@b\nACGT\n+\nI\n")
     assert recs == [(b"a", b"", b"AC", b"II")]
+    # a header character as the file's last byte starts no record (kseq's
+    # ks_getuntil returns -1 when it reads nothing at the end)
+    recs, stats, pk = read(b">a\nACGT\n>")
+    assert recs == [(b"a", b"", b"ACGT", b"")] and stats[:3] == (4, 4, 4)
+    assert abs(stats[3] - 4.0) < 1e-6 and pk == b"ACGT$\0"
+    # gzip input is decompressed (the reference's HAVE_ZLIB build,
+    # src/parser.c:86-114), giving the records of the plain file
+    import gzip
+    text = b">sp|P1 first protein\nMKV\nLLA\n>P2\nACD\n"
+    plain = read(text)
+    f = tmp_path / "in.fa.gz"
+    f.write_bytes(gzip.compress(text))
+    p = L.parasail_sequences_from_file(str(f).encode())
+    assert p and p.contents.l == 2
+    assert [p.contents.seqs[i].seq.s for i in range(2)] == [r[2] for r in plain[0]]
+    L.parasail_sequences_free(p)
     L.parasail_sequences_free(None)
     assert not L.parasail_sequences_from_file(str(tmp_path / "missing.fa").encode())

@@ -620,3 +620,295 @@ def test_resident_batch_single_and_multi(mats):
             assert L.parasail_hip_last_pass_ms(len(order), None, None) == -1
     finally:
         L.parasail_hip_batch_free(h)
+
+
+# ------------------------------------------------ saturation windows
+#
+# The reference's fixed-width kernels report saturation by predicates of
+# their own (align.c restates them); here they are applied, independently,
+# to the exact DP values of a plain Python DP over every cell.
+
+def _dp_values(alg, flags, q, t, M, o, g):
+    """(min of H/E/F over cells and boundaries, max cell H, min cell H,
+    max length count of a cell's H) -- the recurrences of src/sw_stats.c,
+    src/sg_stats.c, src/nw_stats.c cell by cell."""
+    m, n = len(q), len(t)
+    NEGI = -(1 << 30)
+    s1b, s1e, s2b, s2e = flags if alg == O.SG else (0, 0, 0, 0)
+    sw = alg == O.SW
+    Hp = [0] + [0 if (sw or s2b) else -o - (j - 1) * g for j in range(1, n + 1)]
+    Fp = [NEGI] * (n + 1)
+    Lp = [0] * (n + 1)
+    FLp = [0] * (n + 1)
+    vmin = min(Hp)
+    vmax = vminh = vmaxl = 0
+    for i in range(1, m + 1):
+        Hc = [0 if (sw or s1b) else -o - (i - 1) * g] + [0] * n
+        vmin = min(vmin, Hc[0])
+        Lc = [0] * (n + 1)
+        FLc = [0] * (n + 1)
+        Fc = [NEGI] * (n + 1)
+        E, EL = NEGI, 0
+        for j in range(1, n + 1):
+            f_opn, f_ext = Hp[j] - o, Fp[j] - g
+            F = max(f_opn, f_ext)
+            e_opn, e_ext = Hc[j - 1] - o, E - g
+            E = max(e_opn, e_ext)
+            hd = Hp[j - 1] + int(M[q[i - 1], t[j - 1]])
+            H = max(hd, E, F, 0) if sw else max(hd, E, F)
+            fl = (Lp[j] if f_opn > f_ext else FLp[j]) + 1
+            EL = (Lc[j - 1] if e_opn > e_ext else EL) + 1
+            L = Lp[j - 1] + 1 if H == hd else (fl if H == F else EL)
+            if sw and H <= 0:
+                L = 0
+            Hc[j], Fc[j], Lc[j], FLc[j] = H, F, L, fl
+            vmin = min(vmin, H, E, F)
+            vmax = max(vmax, H)
+            vminh = min(vminh, H)
+            vmaxl = max(vmaxl, L)
+        Hp, Fp, Lp, FLp = Hc, Fc, Lc, FLc
+    return vmin, vmax, vminh, vmaxl
+
+
+def _ref_saturates(alg, kind, scan, w, o, mx_max, mx_min, score, vals):
+    vmin, vmax, vminh, vmaxl = vals
+    pos = (1 << (w - 1)) - 1 - mx_max - 1
+    neg = ((-(1 << (w - 1)) + o) if -o < mx_min else (-(1 << (w - 1)) - mx_min)) + 1
+    if alg == O.SW:
+        if not scan and kind != "trace":
+            # biased by INT_MIN(w) (sw_striped_avx2_256_16.c:205-210,350-362);
+            # a count clamps at 2^w - 1 (sw_stats_striped_avx2_256_16.c:440-442)
+            return score > (1 << w) - mx_max - 2 or (kind == "stats" and vmaxl >= (1 << w) - 1)
+        return score > pos or (kind == "stats" and vmaxl > pos)
+    if kind == "stats":
+        return vminh < neg or vmax > pos or vmaxl > pos
+    if scan and kind == "score" and alg == O.NW:
+        return vminh < neg or vmax > pos
+    return vmin < neg or vmax > pos
+
+
+def test_saturation_windows_by_kind_and_strategy(mats):
+    """Every fixed width (8, 16) and sat of SW, NW and SG, score/_stats/
+    _trace, striped and scan: saturation and the saturated outputs follow
+    the reference's kernel of that variant (stats counts included:
+    sg_stats_striped_avx2_256_8.c:474-477, sw_stats_striped_avx2_256_8.c:
+    473-475,645); unsaturated results equal the oracle."""
+    rng = random.Random(77)
+    pm, om = mats["blosum62"]
+    M = om.values()
+    a150 = rseq(rng, 150, weights=RR)
+    b150 = rseq(rng, 150, weights=RR)
+    r300 = rseq(rng, 300, weights=RR)
+    cases = [(a150, b150), (r300, mutate(rng, r300, AA, sub=0.45, indel=0.02)),
+             (rseq(rng, 40), rseq(rng, 45)), ("W" * 14, "W" * 14)]
+    for s1, s2 in cases:
+        q, t = om.map(s1), om.map(s2)
+        for name in ("sw", "nw", "sg"):
+            alg, flags = oracle_alg(name)
+            vals = _dp_values(alg, flags, q, t, M, 11, 1)
+            for kind in ("score", "stats", "trace"):
+                okind = {"score": O.SCORE, "stats": O.STATS, "trace": O.TRACE}[kind]
+                exp, _ = O.align(alg, okind, s1, s2, 11, 1, om, flags)
+                suffix = {"score": "", "stats": "_stats", "trace": "_trace"}[kind]
+                for scan in (0, 1):
+                    strat = "scan" if scan else "striped"
+                    sat_w = [w for w in (8, 16) if not _ref_saturates(
+                        alg, kind, scan, w, 11, om.max, om.min, exp[0], vals)]
+                    for w in ("8", "16", "sat"):
+                        fn = "parasail_%s%s_%s_hip_64_%s" % (name, suffix, strat, w)
+                        got = P.align(fn, s1, s2, 11, 1, pm)
+                        ctx = (fn, len(s1), len(s2), vals, exp)
+                        if w == "sat":
+                            ew = sat_w[0] if sat_w else 32
+                            bits = {8: P.FLAG_BITS_8, 16: P.FLAG_BITS_16, 32: P.FLAG_BITS_32}[ew]
+                            assert got["flag"] & bits and not got["flag"] & P.FLAG_SATURATED, ctx
+                            assert got["score"] == exp[0], ctx
+                            continue
+                        sat = int(w) not in sat_w
+                        assert bool(got["flag"] & P.FLAG_SATURATED) == sat, ctx
+                        if not sat:
+                            assert (got["score"], got["end_query"], got["end_ref"]) == exp[:3], ctx
+                            continue
+                        wi = int(w)
+                        if alg == O.SW and not scan and kind == "score":
+                            sv = (1 << wi) - 1
+                        elif alg == O.SW and not scan and kind == "stats":
+                            sv = 1 << (wi - 1)
+                        else:
+                            sv = 0
+                        assert (got["score"], got["end_query"], got["end_ref"]) == (sv, 0, 0), ctx
+                        if kind == "stats":
+                            assert (got["matches"], got["similar"], got["length"]) == (sv,) * 3, ctx
+    # the case the reference's rule is about: a ~150 aa pair saturates the
+    # 8-bit sg _stats kernel through its counts while its score fits
+    assert _dp_values(O.SG, (1, 1, 1, 1), om.map(a150), om.map(b150), M, 11, 1)[3] > 115
+
+
+def test_batch_stats_sat_widths_follow_counts(mats):
+    """Batched SW/NW/SG _stats and _trace with width sat report, per pair,
+    the width the per-call sat function reports (counts included)."""
+    rng = random.Random(78)
+    pm, om = mats["blosum62"]
+    qs, ts = [], []
+    for k in range(24):
+        a = rseq(rng, rng.randint(20, 320), weights=RR)
+        b = mutate(rng, a, AA, sub=0.4, indel=0.02) if k % 2 else rseq(rng, rng.randint(20, 320))
+        qs.append(a)
+        ts.append(b)
+    for name in ("sw", "nw", "sg", "sg_qx"):
+        for kind, kname in ((1, "_stats"), (2, "_trace")):
+            got = P.align_batch(name, kind, 0, qs, ts, 11, 1, pm)
+            for k in range(len(qs)):
+                one = P.align("parasail_%s%s_striped_hip_64_sat" % (name, kname), qs[k], ts[k],
+                              11, 1, pm)
+                assert int(got["flag"][k]) == one["flag"], (name, kind, k)
+                assert int(got["score"][k]) == one["score"]
+                if kind == 1:
+                    assert (int(got["length"][k]), int(got["matches"][k])) == \
+                        (one["length"], one["matches"])
+
+
+def _concat(seqs):
+    bs = [s.encode() for s in seqs]
+    ln = np.array([len(x) for x in bs], dtype=np.int32)
+    off = np.zeros(len(bs), dtype=np.int64)
+    off[1:] = np.cumsum(ln, dtype=np.int64)[:-1]
+    return b"".join(bs), off, ln
+
+
+def test_batch_int16_overflow_redo(mats):
+    """Config-1-shaped batches with pairs the packed-int16 kernel cannot
+    hold.  SW: with a match score of 120, self pairs of 300 and 700 residues
+    score past the int16 overflow limit 32767 - (max + open + 1) and are
+    redone by the int32 tier after the run-time test of
+    src/sw_striped_avx2_256_16.c:350-362; the 300 one fits the biased 16-bit
+    window (2^16 - max - 2), the 700 one only 32 bits, and width 16 reports
+    it saturated (src/sw_striped_avx2_256_16.c:374-382).  NW/SG: a pair past
+    the provable int16 bound runs on an int32 tier (its sat flag reads 32).
+    Scores and ends equal the oracle through parasail_hip_align_batch and
+    the resident batch; the SW sat flags are the per-call sat function's."""
+    rng = random.Random(90)
+    pb, ob = mats["blosum62"]
+    alpha = "ACDEFGHIKLMNPQRSTVWY"
+    pbig, obig = P.matrix_create(alpha, 120, -10), O.matrix_create(alpha, 120, -10)
+    qs, ts = [], []
+    for _ in range(300):
+        a = rseq(rng, int(min(1200, max(20, rng.lognormvariate(np.log(300), 0.6)))), weights=RR)
+        b = rseq(rng, int(min(1200, max(20, rng.lognormvariate(np.log(300), 0.6)))), weights=RR)
+        qs.append(a)
+        ts.append(b)
+    s300, s700 = rseq(rng, 300), rseq(rng, 700)
+    lng = rseq(rng, 1700, weights=RR)
+    L = _resident_api()
+    I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
+    # SW over the big-match pairs; NW and SG past the int16 bound with a
+    # 1,700-residue pair at extend 10 (every value provably inside int16
+    # needs (m + n) * extend below ~32,000)
+    for name, gap, pm, om in (("sw", 1, pbig, obig), ("nw", 10, pb, ob), ("sg", 10, pb, ob)):
+        alg, fl = oracle_alg(name)
+        qn, tn = list(qs), list(ts)
+        if name == "sw":
+            qn[40], tn[40] = s300, s300
+            qn[170], tn[170] = s700, s700
+        qn[260], tn[260] = lng, mutate(rng, lng, AA, 0.1, 0.02)
+        qcat, qoff, qlen = _concat(qn)
+        tcat, toff, tlen = _concat(tn)
+        h = L.parasail_hip_batch_create(qcat, qoff.ctypes.data_as(I64), qlen.ctypes.data_as(I32),
+                                        tcat, toff.ctypes.data_as(I64), tlen.ctypes.data_as(I32),
+                                        len(qn), pm)
+        assert h
+        try:
+            got = P.align_batch(name, 0, 0, qn, tn, 11, gap, pm)
+            arrs, br = _bufs(len(qn))
+            assert L.parasail_hip_batch_align(h, name.encode(), 11, gap, ctypes.byref(br)) == 0
+        finally:
+            L.parasail_hip_batch_free(h)
+        for key in ("score", "end_query", "end_ref", "flag"):
+            assert (arrs[key] == got[key]).all(), (name, key)
+        for k in range(len(qn)):
+            exp, _ = O.align(alg, O.SCORE, qn[k], tn[k], 11, gap, om, fl)
+            assert (got["score"][k], got["end_query"][k], got["end_ref"][k]) == exp[:3], \
+                (name, k, len(qn[k]), len(tn[k]))
+        if name != "sw":
+            assert got["flag"][260] & P.FLAG_BITS_32
+            continue
+        bits = P.FLAG_BITS_8 | P.FLAG_BITS_16 | P.FLAG_BITS_32
+        ovf = 32767 - (120 + 11 + 1)
+        assert ovf < got["score"][40] <= 65535 - 122 < got["score"][170]
+        for k in (40, 170, 260):
+            one = P.align("parasail_sw_striped_hip_64_sat", qn[k], tn[k], 11, gap, pm)
+            assert int(got["flag"][k]) & bits == one["flag"] & bits, k
+        assert got["flag"][40] & P.FLAG_BITS_16 and got["flag"][170] & P.FLAG_BITS_32
+        g16 = P.align_batch(name, 0, 16, qn, tn, 11, gap, pm)
+        assert g16["flag"][170] & P.FLAG_SATURATED
+        assert (g16["score"][170], g16["end_query"][170], g16["end_ref"][170]) == (65535, 0, 0)
+        assert not g16["flag"][40] & P.FLAG_SATURATED and g16["score"][40] == got["score"][40]
+
+
+def test_file_to_resident_batch(tmp_path, fixtures, mats):
+    """FASTA records (one file plain, one gzip) read by
+    parasail_sequences_from_file, packed by parasail_sequences_pack, split
+    at the '$' separators into offsets and lengths, made an HBM-resident
+    batch (record i of one file against record i of the other) and aligned:
+    scores and ends equal the oracle's."""
+    import gzip
+    pm, om = mats["blosum62"]
+    recs = fixtures["fasta"]["test_P0_10.fasta"]
+    fa = tmp_path / "a.fa"
+    fb = tmp_path / "b.fa.gz"
+    fa.write_text("".join(">%s\n%s\n" % (nm, s) for nm, s in recs))
+    fb.write_bytes(gzip.compress("".join(">%s x\n%s\n" % (nm, s[::-1][:300])
+                                         for nm, s in recs).encode()))
+    L = _resident_api()
+
+    class Str(ctypes.Structure):
+        _fields_ = [("l", ctypes.c_size_t), ("s", ctypes.c_char_p)]
+
+    class Seq(ctypes.Structure):
+        _fields_ = [("name", Str), ("comment", Str), ("seq", Str), ("qual", Str)]
+
+    class Seqs(ctypes.Structure):
+        _fields_ = [("seqs", ctypes.POINTER(Seq)), ("l", ctypes.c_size_t),
+                    ("characters", ctypes.c_size_t), ("shortest", ctypes.c_size_t),
+                    ("longest", ctypes.c_size_t), ("mean", ctypes.c_float),
+                    ("stddev", ctypes.c_float)]
+
+    L.parasail_sequences_from_file.restype = ctypes.POINTER(Seqs)
+    L.parasail_sequences_from_file.argtypes = [ctypes.c_char_p]
+    L.parasail_sequences_pack.restype = ctypes.c_void_p
+    L.parasail_sequences_pack.argtypes = [ctypes.POINTER(Seqs), ctypes.POINTER(ctypes.c_size_t)]
+    L.parasail_sequences_free.argtypes = [ctypes.POINTER(Seqs)]
+
+    def packed(path):
+        p = L.parasail_sequences_from_file(str(path).encode())
+        assert p
+        n = ctypes.c_size_t()
+        buf = L.parasail_sequences_pack(p, ctypes.byref(n))
+        data = ctypes.string_at(buf, n.value)
+        L._libc_free(buf)
+        L.parasail_sequences_free(p)
+        # record i starts after the i-th '$'
+        ends = [i for i, c in enumerate(data) if c == ord("$")]
+        starts = [0] + [e + 1 for e in ends[:-1]]
+        return data, np.array(starts, dtype=np.int64), \
+            np.array([e - s for s, e in zip(starts, ends)], dtype=np.int32)
+
+    qd, qoff, qlen = packed(fa)
+    td, toff, tlen = packed(fb)
+    assert len(qlen) == len(tlen) == len(recs) and (qlen > 0).all() and (tlen > 0).all()
+    I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
+    h = L.parasail_hip_batch_create(qd, qoff.ctypes.data_as(I64), qlen.ctypes.data_as(I32), td,
+                                    toff.ctypes.data_as(I64), tlen.ctypes.data_as(I32),
+                                    len(qlen), pm)
+    assert h
+    try:
+        for name in ("sw", "nw", "sg"):
+            alg, fl = oracle_alg(name)
+            arrs, br = _bufs(len(qlen))
+            assert L.parasail_hip_batch_align(h, name.encode(), 11, 1, ctypes.byref(br)) == 0
+            for k, (_, s) in enumerate(recs):
+                exp, _ = O.align(alg, O.SCORE, s, s[::-1][:300], 11, 1, om, fl)
+                assert (arrs["score"][k], arrs["end_query"][k], arrs["end_ref"][k]) == exp[:3], \
+                    (name, k)
+    finally:
+        L.parasail_hip_batch_free(h)
