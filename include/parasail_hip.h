@@ -384,6 +384,27 @@ int parasail_hip_align_batch(const char *alg_name, int kind, int width, const ch
                              int gap, const parasail_matrix_t *matrix,
                              parasail_hip_batch_result_t *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * memory -- code under HIP graph capture, serving loops that must not call
+ * hipMalloc: the variant of parasail_hip_align_batch at width 32 (exact
+ * int32 results, kind 0 score or 1 stats) whose inputs and outputs are in
+ * device memory.  s1/s2 are the residues as given (mapped on the device
+ * through matrix->mapper) with device offsets and lengths; every array of
+ * out is a device array of npairs ints (any may be NULL; cigars must be
+ * NULL).  The work is queued on `stream` (a hipStream_t; NULL: the null
+ * stream) and the call returns without waiting.  It allocates nothing:
+ * device scratch comes from a static 64 MiB array per GPU, which limits
+ * max_len, the longest s2 (a bound the caller gives; a pair whose s2 is
+ * longer reports PARASAIL_FLAG_INVALID), to 8,248,832 residues for scores
+ * and 2,062,208 for stats; the call fails with a message past that.  The matrix is square, of at most 128 letters.  Calls
+ * that overlap on one GPU share the array and race.  0, or -1. */
+int parasail_hip_align_batch_noalloc(const char *alg_name, int kind, const char *s1,
+                                     const int64_t *s1_off, const int *s1_len, const char *s2,
+                                     const int64_t *s2_off, const int *s2_len, int npairs,
+                                     int max_len, int open, int gap,
+                                     const parasail_matrix_t *matrix,
+                                     parasail_hip_batch_result_t *out, void *stream);
+
 /* Banded global alignment (parasail_nw_banded semantics, band k) of npairs
  * pairs in one device pass; score and end positions into out (the other
  * arrays of out are zeroed).  0, or -1 with a message. */

@@ -26,6 +26,9 @@ struct DPParams {
     phip_out_t *out;       // device
     int8_t *trace;         // device, row-major per pair (kind == TRACE)
     const int64_t *trace_off;
+    const int *mapper;     // k_simple: device, 256 entries; q/t then hold the
+                           // residues as given, mapped cell by cell (NULL:
+                           // q/t hold codes)
 };
 
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }

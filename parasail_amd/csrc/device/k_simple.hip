@@ -8,6 +8,8 @@
 // kernels and the tier for _stats/_trace; it restates the recurrences of
 // src/sw.c:131-158, src/sg.c:147-235, src/sw_stats.c:204-261,
 // src/sg_stats.c:227-309 and src/sg_trace.c:144-150 cell for cell.
+#include <string.h>
+
 #include "common.h"
 
 namespace {
@@ -57,6 +59,16 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
     const int msize = p.msize;
     const bool stats = kind == PHIP_STATS, trace = kind == PHIP_TRACE;
     int8_t *HT = trace ? p.trace + p.trace_off[pr] : nullptr;
+    const int *mapper = p.mapper;
+    auto code = [mapper](const uint8_t *s, int k) -> int { return mapper ? mapper[s[k]] : s[k]; };
+    if (n > maxn || n <= 0 || m <= 0) {
+        // outside the row scratch this launch was sized for
+        phip_out_t o;
+        memset(&o, 0, sizeof(o));
+        o.status = PHIP_ST_ERR;
+        p.out[pr] = o;
+        return;
+    }
 
     const size_t plane = (size_t)(maxn + 1) * stride;
     Rows r;
@@ -95,8 +107,9 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
     }
 
     for (int i = 1; i <= m; ++i) {
+        const int qi = s1 ? code(s1, i - 1) : 0;
         const int *matrow = p.is_pssm ? mat + (size_t)msize * (i - 1)
-                                      : mat + (size_t)msize * s1[i - 1];
+                                      : mat + (size_t)msize * qi;
         int NH = at(r.H, 0, stride, slot);
         int NHM = 0, NHS = 0, NHL = 0;
         int WH = (alg == PHIP_SW || s1_beg) ? 0 : -open - (i - 1) * gap;
@@ -117,7 +130,8 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
             const int F = imax(F_opn, F_ext);
             const int E_opn = WH - open, E_ext = E - gap;
             E = imax(E_opn, E_ext);
-            const int sub = matrow[s2[j - 1]];
+            const int tj = code(s2, j - 1);
+            const int sub = matrow[tj];
             const int H_dag = NWH + sub;
             int H = imax(H_dag, E);
             H = imax(H, F);
@@ -133,7 +147,7 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
                 if (E_opn > E_ext) { EM = WHM; ES = WHS; EL = WHL; }
                 EL += 1;
                 if (H == H_dag) {
-                    WHM = NWM + (s1[i - 1] == s2[j - 1]);
+                    WHM = NWM + (qi == tj);
                     WHS = NWS + (sub > 0);
                     WHL = NWL + 1;
                 } else if (H == F) {
