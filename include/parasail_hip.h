@@ -277,6 +277,20 @@ int parasail_result_get_matches(const parasail_result_t *const result);
 int parasail_result_get_similar(const parasail_result_t *const result);
 int parasail_result_get_length(const parasail_result_t *const result);
 int *parasail_result_get_trace_table(const parasail_result_t *const result);
+int *parasail_result_get_trace_ins_table(const parasail_result_t *const result);
+int *parasail_result_get_trace_del_table(const parasail_result_t *const result);
+int *parasail_result_get_score_table(const parasail_result_t *const result);
+int *parasail_result_get_matches_table(const parasail_result_t *const result);
+int *parasail_result_get_similar_table(const parasail_result_t *const result);
+int *parasail_result_get_length_table(const parasail_result_t *const result);
+int *parasail_result_get_score_row(const parasail_result_t *const result);
+int *parasail_result_get_matches_row(const parasail_result_t *const result);
+int *parasail_result_get_similar_row(const parasail_result_t *const result);
+int *parasail_result_get_length_row(const parasail_result_t *const result);
+int *parasail_result_get_score_col(const parasail_result_t *const result);
+int *parasail_result_get_matches_col(const parasail_result_t *const result);
+int *parasail_result_get_similar_col(const parasail_result_t *const result);
+int *parasail_result_get_length_col(const parasail_result_t *const result);
 
 /* striped or scan table -> row-major int table (src/striped_unwind.c:
  * 5-26, :56-111; parasail/memory.h:137-141): `array` is the result's trace
@@ -331,6 +345,32 @@ int *parasail_striped_unwind(int lena, int lenb, parasail_result_t *result, void
     PARASAIL_HIP_DECL_S(alg, _trace, scan)
 
 PARASAIL_HIP_ALGS(PARASAIL_HIP_DECL_ALG)
+
+/* The _table and _rowcol forms, replacing parasail_<ALG>[_stats]_{table,
+ * rowcol}_<STRATEGY>[_profile]_<isa>_<bits>_<WIDTH> (built from each kernel
+ * with -DPARASAIL_TABLE / -DPARASAIL_ROWCOL, CMakeLists.txt:800-861): the
+ * result also carries every cell's H (and, for _stats, the matches,
+ * similar and length counts of that H) as s1Len x s2Len row-major tables
+ * (parasail_result_new_table1/table3, src/memory.c:171-229), or only the
+ * DP's last row (s2Len entries) and last column (s1Len entries)
+ * (parasail_result_new_rowcol1/rowcol3, src/memory.c:188-275); read them
+ * with parasail_result_get_score_table, _get_score_row, ... */
+#define PARASAIL_HIP_DECL_TS(alg, kind, tab, strat)                             \
+    PARASAIL_HIP_DECL_W(alg, kind##_##tab, strat, 8)                            \
+    PARASAIL_HIP_DECL_W(alg, kind##_##tab, strat, 16)                           \
+    PARASAIL_HIP_DECL_W(alg, kind##_##tab, strat, 32)                           \
+    PARASAIL_HIP_DECL_W(alg, kind##_##tab, strat, sat)
+#define PARASAIL_HIP_DECL_TALG(alg)                                             \
+    PARASAIL_HIP_DECL_TS(alg, , table, striped)                                 \
+    PARASAIL_HIP_DECL_TS(alg, , table, scan)                                    \
+    PARASAIL_HIP_DECL_TS(alg, , rowcol, striped)                                \
+    PARASAIL_HIP_DECL_TS(alg, , rowcol, scan)                                   \
+    PARASAIL_HIP_DECL_TS(alg, _stats, table, striped)                           \
+    PARASAIL_HIP_DECL_TS(alg, _stats, table, scan)                              \
+    PARASAIL_HIP_DECL_TS(alg, _stats, rowcol, striped)                          \
+    PARASAIL_HIP_DECL_TS(alg, _stats, rowcol, scan)
+
+PARASAIL_HIP_ALGS(PARASAIL_HIP_DECL_TALG)
 
 /* profile creators, replacing parasail_profile_create[_stats]_avx_256_<W>
  * (src/memory_avx2.c:81-132, :240-253, :255-318).  The profile keeps the

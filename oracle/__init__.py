@@ -55,6 +55,10 @@ def lib():
             P(ctypes.c_int), ctypes.c_int, P(ctypes.c_int), ctypes.c_int,
             ctypes.c_int, ctypes.c_int, P(OrcMatrix), P(OrcResult),
             P(ctypes.c_int8)]
+        _lib.orc_align_tables.argtypes = [ctypes.c_int] * 6 + [
+            P(ctypes.c_int), ctypes.c_int, P(ctypes.c_int), ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, P(OrcMatrix), P(OrcResult),
+            P(ctypes.c_int8), P(P(ctypes.c_int))]
         _lib.orc_nw_banded.argtypes = [
             P(ctypes.c_int), ctypes.c_int, P(ctypes.c_int), ctypes.c_int,
             ctypes.c_int, ctypes.c_int, ctypes.c_int, P(OrcMatrix), P(OrcResult)]
@@ -199,6 +203,28 @@ def align(alg, kind, s1, s2, open_, gap, matrix, sg_flags=(0, 0, 0, 0)):
     if rc != 0:
         return None, None
     return res.as_tuple(), (None if tr is None else tr.reshape(m, len(t)))
+
+
+def align_tables(alg, kind, s1, s2, open_, gap, matrix, sg_flags=(0, 0, 0, 0)):
+    """orc_align with the PARASAIL_TABLE outputs: (result tuple, tables) where
+    tables is an (m, n) int32 array of every cell's H, or for STATS a list of
+    four (H, matches, similar, length)."""
+    q = None if s1 is None else matrix.map(s1)
+    t = matrix.map(s2)
+    m = matrix.length if matrix.is_pssm else len(q)
+    planes = 4 if kind == STATS else 1
+    tabs = [np.zeros(m * len(t), dtype=np.int32) for _ in range(planes)]
+    ptrs = (ctypes.POINTER(ctypes.c_int) * 4)(*([_iptr(x) for x in tabs] +
+                                                 [None] * (4 - planes)))
+    res = OrcResult()
+    rc = lib().orc_align_tables(alg, kind, *sg_flags,
+                                None if q is None else _iptr(q), 0 if q is None else len(q),
+                                _iptr(t), len(t), open_, gap, matrix.ptr, ctypes.byref(res),
+                                None, ptrs)
+    if rc != 0:
+        return None, None
+    tabs = [x.reshape(m, len(t)) for x in tabs]
+    return res.as_tuple(), (tabs if kind == STATS else tabs[0])
 
 
 def nw_banded(s1, s2, open_, gap, k, matrix):

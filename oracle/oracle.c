@@ -236,6 +236,17 @@ int orc_align(int alg, int kind, int s1_beg, int s1_end, int s2_beg, int s2_end,
               const int *s1, int m, const int *s2, int n, int open, int gap,
               const orc_matrix_t *mx, orc_result_t *res, int8_t *trace)
 {
+    return orc_align_tables(alg, kind, s1_beg, s1_end, s2_beg, s2_end, s1, m, s2, n, open, gap,
+                            mx, res, trace, NULL);
+}
+
+/* As orc_align; with tabs non-NULL also the PARASAIL_TABLE outputs
+ * (sw.c:159-170, sg_stats.c:258-263): every cell's H into tabs[0] and, for
+ * ORC_STATS, its matches/similar/length into tabs[1..3], m*n ints row-major. */
+int orc_align_tables(int alg, int kind, int s1_beg, int s1_end, int s2_beg, int s2_end,
+                     const int *s1, int m, const int *s2, int n, int open, int gap,
+                     const orc_matrix_t *mx, orc_result_t *res, int8_t *trace, int **tabs)
+{
     /* PARASAIL_CHECK_* (sw.c:47-56) */
     if (!s2 || n <= 0 || open < 0 || gap < 0 || !mx || !res) return -1;
     const int pssm = mx->type == 1;
@@ -321,6 +332,11 @@ int orc_align(int alg, int kind, int s1_beg, int s1_end, int s2_beg, int s2_end,
             }
             H[j] = WH;
             HM[j] = WHM; HS[j] = WHS; HL[j] = WHL;
+            if (tabs) {
+                const long at = (long)(i - 1) * n + (j - 1);
+                tabs[0][at] = WH;
+                if (kind == ORC_STATS) { tabs[1][at] = WHM; tabs[2][at] = WHS; tabs[3][at] = WHL; }
+            }
 
             if (alg == ORC_SW) {
                 /* sw.c:150-158: first strictly better cell, ties -> smaller j */

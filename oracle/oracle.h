@@ -86,6 +86,11 @@ void orc_map(const orc_matrix_t *mx, const char *s, int len, int *out);
 int orc_align(int alg, int kind, int s1_beg, int s1_end, int s2_beg, int s2_end,
               const int *s1, int m, const int *s2, int n, int open, int gap,
               const orc_matrix_t *mx, orc_result_t *res, int8_t *trace);
+/* orc_align plus the DP tables: tabs[0] every cell's H, tabs[1..3] (stats)
+ * its matches, similar, length; m*n ints each, row-major */
+int orc_align_tables(int alg, int kind, int s1_beg, int s1_end, int s2_beg, int s2_end,
+                     const int *s1, int m, const int *s2, int n, int open, int gap,
+                     const orc_matrix_t *mx, orc_result_t *res, int8_t *trace, int **tabs);
 
 /* src/nw_banded.c:29-189 */
 int orc_nw_banded(const int *s1, int m, const int *s2, int n, int open, int gap,

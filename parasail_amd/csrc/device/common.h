@@ -29,6 +29,8 @@ struct DPParams {
     const int *mapper;     // k_simple: device, 256 entries; q/t then hold the
                            // residues as given, mapped cell by cell (NULL:
                            // q/t hold codes)
+    int tab;               // one-pair launches: 1 DP tables, 2 last row and
+    int *tabs[4];          // column, of score (and matches, similar, length)
 };
 
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }

@@ -62,12 +62,17 @@ struct Dp32Cfg {
     int nwaves;
     int8_t *trace;
     phip_out_t *out;
+    int *dtab[4];  // TAB: score, matches, similar, length tables of a one-pair launch
 };
 
-// KIND: PHIP_SCORE, PHIP_STATS or PHIP_TRACE.  EXT: also the exact extrema of H, E and F over the pair's cells and boundaries, the
-// smallest cell H and (_stats) the largest length count of a cell's H, for
-// the caller's saturation test (k_simple's `check`).
-template <int R, int KIND, int ALG, bool EXT>
+// KIND: PHIP_SCORE, PHIP_STATS or PHIP_TRACE.  EXT: also the exact extrema
+// of H, E and F over the pair's cells and boundaries, the smallest cell H
+// and (_stats) the largest length count of a cell's H, for the caller's
+// saturation test (k_simple's `check`).  TAB (one pair per launch): 1 writes
+// every cell's H (and, for _stats, its H's counts) row-major into dtab[],
+// 2 only the last row (n entries) followed by the last column (m entries),
+// the PARASAIL_TABLE / PARASAIL_ROWCOL outputs of src/sw.c:159-170.
+template <int R, int KIND, int ALG, bool EXT, int TAB>
 __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, const int2 *tab,
                                          const int lane)
 {
@@ -191,6 +196,24 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
                     vmax = max(vmax, h);
                     vminh = min(vminh, h);
                     if (STATS) vmaxl = max(vmaxl, HS[r] & 0xfff);
+                }
+                if (TAB && r < nvalid) {
+                    const int i = t * R + r;
+                    int64_t at = -1;
+                    if (TAB == 1) at = (int64_t)i * n + j;
+                    else if (i == m - 1) at = j;
+                    for (int rep = 0; rep < (TAB == 2 ? 2 : 1); ++rep) {
+                        if (at >= 0) {
+                            c.dtab[0][at] = h;
+                            if (STATS) {
+                                const uint32_t w = (uint32_t)HS[r];
+                                c.dtab[1][at] = (int)(w >> 22);
+                                c.dtab[2][at] = (int)((w >> 12) & 0x3ff);
+                                c.dtab[3][at] = (int)(w & 0xfff);
+                            }
+                        }
+                        at = (TAB == 2 && j == n - 1) ? (int64_t)n + i : -1;
+                    }
                 }
                 t1 = t1n;
             }
@@ -354,7 +377,7 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
 
 // LDS: (size+1) x size entries {substitution + open, statistics increment},
 // the last row for query rows past the end.
-template <int KIND, int ALG, bool EXT>
+template <int KIND, int ALG, bool EXT, int TAB>
 __global__ void __launch_bounds__(256) k_dp32(Dp32Cfg c)
 {
     extern __shared__ int2 tab[];
@@ -375,20 +398,20 @@ __global__ void __launch_bounds__(256) k_dp32(Dp32Cfg c)
     const Dp32Wave wd = c.waves[w];
     const int lane = threadIdx.x & 63;
     switch (wd.R) {
-    case 4: dp32_run<4, KIND, ALG, EXT>(c, wd, tab, lane); break;
-    case 8: dp32_run<8, KIND, ALG, EXT>(c, wd, tab, lane); break;
-    case 12: dp32_run<12, KIND, ALG, EXT>(c, wd, tab, lane); break;
-    case 16: dp32_run<16, KIND, ALG, EXT>(c, wd, tab, lane); break;
-    case 20: dp32_run<20, KIND, ALG, EXT>(c, wd, tab, lane); break;
+    case 4: dp32_run<4, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
+    case 8: dp32_run<8, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
+    case 12: dp32_run<12, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
+    case 16: dp32_run<16, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
+    case 20: dp32_run<20, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
     default:
         // the finer shapes of per-call alignments (no traceback)
         if constexpr (KIND != PHIP_TRACE) {
             switch (wd.R) {
-            case 2: dp32_run<2, KIND, ALG, EXT>(c, wd, tab, lane); break;
-            case 3: dp32_run<3, KIND, ALG, EXT>(c, wd, tab, lane); break;
-            case 5: dp32_run<5, KIND, ALG, EXT>(c, wd, tab, lane); break;
-            case 6: dp32_run<6, KIND, ALG, EXT>(c, wd, tab, lane); break;
-            default: dp32_run<10, KIND, ALG, EXT>(c, wd, tab, lane); break;
+            case 2: dp32_run<2, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
+            case 3: dp32_run<3, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
+            case 5: dp32_run<5, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
+            case 6: dp32_run<6, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
+            default: dp32_run<10, KIND, ALG, EXT, TAB>(c, wd, tab, lane); break;
             }
         }
         break;
@@ -731,19 +754,36 @@ int dp32_launch(const DPParams &dp, const Dp32Plan &P, const char *region, hipSt
     const dim3 grid((P.nwaves + 3) / 4), block(256);
     typedef void (*Fn)(Dp32Cfg);
     static const Fn fns[6][3] = {
-        {k_dp32<PHIP_SCORE, PHIP_SW, false>, k_dp32<PHIP_SCORE, PHIP_NW, false>,
-         k_dp32<PHIP_SCORE, PHIP_SG, false>},
-        {k_dp32<PHIP_SCORE, PHIP_SW, true>, k_dp32<PHIP_SCORE, PHIP_NW, true>,
-         k_dp32<PHIP_SCORE, PHIP_SG, true>},
-        {k_dp32<PHIP_STATS, PHIP_SW, false>, k_dp32<PHIP_STATS, PHIP_NW, false>,
-         k_dp32<PHIP_STATS, PHIP_SG, false>},
-        {k_dp32<PHIP_TRACE, PHIP_SW, false>, k_dp32<PHIP_TRACE, PHIP_NW, false>,
-         k_dp32<PHIP_TRACE, PHIP_SG, false>},
-        {k_dp32<PHIP_STATS, PHIP_SW, true>, k_dp32<PHIP_STATS, PHIP_NW, true>,
-         k_dp32<PHIP_STATS, PHIP_SG, true>},
-        {k_dp32<PHIP_TRACE, PHIP_SW, true>, k_dp32<PHIP_TRACE, PHIP_NW, true>,
-         k_dp32<PHIP_TRACE, PHIP_SG, true>}};
+        {k_dp32<PHIP_SCORE, PHIP_SW, false, 0>, k_dp32<PHIP_SCORE, PHIP_NW, false, 0>,
+         k_dp32<PHIP_SCORE, PHIP_SG, false, 0>},
+        {k_dp32<PHIP_SCORE, PHIP_SW, true, 0>, k_dp32<PHIP_SCORE, PHIP_NW, true, 0>,
+         k_dp32<PHIP_SCORE, PHIP_SG, true, 0>},
+        {k_dp32<PHIP_STATS, PHIP_SW, false, 0>, k_dp32<PHIP_STATS, PHIP_NW, false, 0>,
+         k_dp32<PHIP_STATS, PHIP_SG, false, 0>},
+        {k_dp32<PHIP_TRACE, PHIP_SW, false, 0>, k_dp32<PHIP_TRACE, PHIP_NW, false, 0>,
+         k_dp32<PHIP_TRACE, PHIP_SG, false, 0>},
+        {k_dp32<PHIP_STATS, PHIP_SW, true, 0>, k_dp32<PHIP_STATS, PHIP_NW, true, 0>,
+         k_dp32<PHIP_STATS, PHIP_SG, true, 0>},
+        {k_dp32<PHIP_TRACE, PHIP_SW, true, 0>, k_dp32<PHIP_TRACE, PHIP_NW, true, 0>,
+         k_dp32<PHIP_TRACE, PHIP_SG, true, 0>}};
+    // tables and last row/column (one pair; extrema always computed)
+    static const Fn tabs[2][2][3] = {
+        {{k_dp32<PHIP_SCORE, PHIP_SW, true, 1>, k_dp32<PHIP_SCORE, PHIP_NW, true, 1>,
+          k_dp32<PHIP_SCORE, PHIP_SG, true, 1>},
+         {k_dp32<PHIP_STATS, PHIP_SW, true, 1>, k_dp32<PHIP_STATS, PHIP_NW, true, 1>,
+          k_dp32<PHIP_STATS, PHIP_SG, true, 1>}},
+        {{k_dp32<PHIP_SCORE, PHIP_SW, true, 2>, k_dp32<PHIP_SCORE, PHIP_NW, true, 2>,
+          k_dp32<PHIP_SCORE, PHIP_SG, true, 2>},
+         {k_dp32<PHIP_STATS, PHIP_SW, true, 2>, k_dp32<PHIP_STATS, PHIP_NW, true, 2>,
+          k_dp32<PHIP_STATS, PHIP_SG, true, 2>}}};
     if (dp.alg < 0 || dp.alg > 2 || dp.kind < 0 || dp.kind > 2) return -1;
+    if (dp.tab) {
+        if (dp.tab < 1 || dp.tab > 2 || dp.kind == PHIP_TRACE) return -1;
+        for (int k = 0; k < 4; ++k) c.dtab[k] = dp.tabs[k];
+        hipLaunchKernelGGL(tabs[dp.tab - 1][dp.kind][dp.alg], grid, block, lds, st, c);
+        HIP_OK(hipGetLastError());
+        return 0;
+    }
     const int row = dp.kind == PHIP_SCORE ? (dp.check ? 1 : 0)
                     : dp.kind == PHIP_STATS ? (dp.check ? 4 : 2) : (dp.check ? 5 : 3);
     hipLaunchKernelGGL(fns[row][dp.alg], grid, block, lds, st, c);

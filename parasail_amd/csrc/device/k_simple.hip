@@ -166,6 +166,24 @@ __device__ void simple_pair(const DPParams &p, int pr, int *scratch, int stride,
                 if (alg == PHIP_SW && H == 0) code &= 120;
                 HT[(size_t)(i - 1) * n + (j - 1)] = (int8_t)code;
             }
+            if (p.tab) {
+                // one-pair launch: the DP table (src/sw.c:159-170) or its
+                // last row, then its last column
+                int64_t at = -1;
+                if (p.tab == 1) at = (int64_t)(i - 1) * n + (j - 1);
+                else if (i == m) at = j - 1;
+                for (int rep = 0; rep < (p.tab == 2 ? 2 : 1); ++rep) {
+                    if (at >= 0) {
+                        p.tabs[0][at] = H;
+                        if (stats) {
+                            p.tabs[1][at] = WHM;
+                            p.tabs[2][at] = WHS;
+                            p.tabs[3][at] = WHL;
+                        }
+                    }
+                    at = (p.tab == 2 && j == n) ? (int64_t)n + (i - 1) : -1;
+                }
+            }
             if (p.check) {
                 vmin = imin(vmin, imin(H, imin(E, F)));
                 vmax = imax(vmax, H);

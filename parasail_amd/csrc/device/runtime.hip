@@ -670,6 +670,87 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
     return rc;
 }
 
+extern "C" int phip_align_tables(const phip_batch_t *b, int tab, phip_out_t *out, int **dst)
+{
+    if (phip_init() != 0) return -1;
+    if (!b || !out || !dst || b->npairs != 1 || (tab != 1 && tab != 2) ||
+        (b->kind != PHIP_SCORE && b->kind != PHIP_STATS) || (b->kind == PHIP_STATS && !b->q)) {
+        phip_set_error("phip_align_tables: bad arguments");
+        return -1;
+    }
+    const int m = b->qlen[0], n = b->tlen[0];
+    const int nplanes = b->kind == PHIP_STATS ? 4 : 1;
+    const size_t per = tab == 1 ? (size_t)m * n : (size_t)m + n;
+    const int64_t zero = 0;
+    HostBatch hb;
+    hb.np = 1;
+    hb.qlen = &m;
+    hb.tlen = &n;
+    hb.qoff = &zero;
+    hb.toff = &zero;
+    hb.mx = &b->matrix;
+    const std::vector<int> pairs(1, 0);
+    const int32_t status = PHIP_ST_W32;
+    Dp32Plan P;
+    // the int32 lane chain when the pair fits it, else the one-lane tier
+    bool wave = b->q && !b->matrix.is_pssm && b->matrix.size <= 32 &&
+                (b->kind != PHIP_STATS || (std::min(m, n) <= 1023 && m + n <= 4095)) &&
+                dp32_plan(hb, pairs, &status, &zero, P, 64, true);
+    pthread_mutex_lock(&g_lock);
+    int rc = -1;
+    phip_resident_t *r = resident_create(b);
+    char *tabs = nullptr, *plan = nullptr;
+    int *scratch = nullptr;
+    do {
+        if (!r) break;
+        tabs = (char *)dev_alloc(per * nplanes * sizeof(int));
+        if (!tabs) break;
+        DPParams dp = make_params(r, b->alg, b->kind, b->sg_flags, b->open, b->gap,
+                                  b->want_sat_check);
+        dp.tab = tab;
+        for (int k = 0; k < 4; ++k)
+            dp.tabs[k] = k < nplanes ? (int *)(tabs + (size_t)k * per * sizeof(int)) : nullptr;
+        if (hipEventRecord(g_ev0, g_stream) != hipSuccess) break;
+        if (wave) {
+            plan = (char *)dev_alloc(P.bytes);
+            if (!plan) break;
+            if (hipMemcpyAsync(plan, P.image.data(), P.bytes, hipMemcpyHostToDevice, g_stream) !=
+                hipSuccess) {
+                phip_set_error("phip_align_tables: plan upload failed");
+                break;
+            }
+            if (dp32_launch(dp, P, plan, g_stream) != 0) break;
+        } else {
+            const int narr = b->kind == PHIP_STATS ? 8 : 2;
+            scratch = (int *)dev_alloc((size_t)(n + 1) * narr * sizeof(int) * 64);
+            if (!scratch) break;
+            hipLaunchKernelGGL(k_simple, dim3(1), dim3(64), 0, g_stream, dp, 0, 1,
+                               (const int32_t *)nullptr, (const int *)nullptr, scratch, 64, n);
+            if (hipGetLastError() != hipSuccess) {
+                phip_set_error("phip_align_tables: launch failed");
+                break;
+            }
+        }
+        if (hipEventRecord(g_ev1, g_stream) != hipSuccess) break;
+        if (resident_fetch(r, out, nullptr) != 0) break;
+        bool ok = true;
+        for (int k = 0; k < nplanes; ++k)
+            ok &= hipMemcpy(dst[k], tabs + (size_t)k * per * sizeof(int), per * sizeof(int),
+                            hipMemcpyDeviceToHost) == hipSuccess;
+        if (!ok) {
+            phip_set_error("phip_align_tables: download failed");
+            break;
+        }
+        rc = 0;
+    } while (0);
+    if (scratch) dev_free(scratch);
+    if (plan) dev_free(plan);
+    if (tabs) dev_free(tabs);
+    if (r) resident_free(r);
+    pthread_mutex_unlock(&g_lock);
+    return rc;
+}
+
 extern "C" int phip_align_cigars(const phip_batch_t *b, const uint8_t *qraw, const uint8_t *traw,
                                  phip_out_t *out, phip_cigars_t *cig)
 {

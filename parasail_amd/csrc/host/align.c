@@ -126,12 +126,12 @@ static int sg_result_flags(int sg_flags)
     return f;
 }
 
-parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kind, int width,
-                               int scan, const char *s1, int s1Len, const char *s2, int s2Len,
-                               int open, int gap, const parasail_matrix_t *matrix,
-                               const parasail_profile_t *profile)
+/* tab: 0, or 1 for the _table and 2 for the _rowcol form of the function
+ * (src/memory.c:171-275 results, src/sw.c:159-170 values) */
+static parasail_result_t *align_ex(const char *fname, int alg, int sg_flags, int kind, int width,
+                                   int scan, int tab, const char *s1, int s1Len, const char *s2,
+                                   int s2Len, int open, int gap, const parasail_matrix_t *matrix)
 {
-    (void)profile;
     CHECK_NULL(fname, s2);
     CHECK_GT0(fname, s2Len);
     CHECK_GE0(fname, open);
@@ -192,17 +192,63 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
     b.tlen = &tl;
     phip_out_t o;
     int8_t *tr = NULL;
-    if (kind == PHIP_TRACE) {
-        tr = (int8_t *)malloc((size_t)m * s2Len);
-        if (!tr) { free(q); free(t); return NULL; }
+    parasail_result_t *rt = NULL; /* the table result, made before the call */
+    int *rcbuf = NULL;            /* _rowcol: last row then last column, per plane */
+    int rc;
+    if (tab) {
+        const int nplanes = kind == PHIP_STATS ? 4 : 1;
+        rt = phost_result_new_tables(m, s2Len, kind == PHIP_STATS, tab == 2);
+        if (tab == 2) rcbuf = (int *)malloc(sizeof(int) * (size_t)nplanes * (m + s2Len));
+        if (!rt || (tab == 2 && !rcbuf)) {
+            if (rt) parasail_result_free(rt);
+            free(q); free(t); free(rcbuf);
+            return NULL;
+        }
+        int *dst[4] = {NULL, NULL, NULL, NULL};
+        for (int k = 0; k < nplanes; ++k) {
+            if (tab == 2) dst[k] = rcbuf + (size_t)k * (m + s2Len);
+            else if (kind != PHIP_STATS) dst[k] = rt->tables->score_table;
+            else dst[k] = k == 0 ? rt->stats->tables->score_table
+                          : k == 1 ? rt->stats->tables->matches_table
+                          : k == 2 ? rt->stats->tables->similar_table
+                                   : rt->stats->tables->length_table;
+        }
+        rc = phip_align_tables(&b, tab, &o, dst);
+    } else {
+        if (kind == PHIP_TRACE) {
+            tr = (int8_t *)malloc((size_t)m * s2Len);
+            if (!tr) { free(q); free(t); return NULL; }
+        }
+        rc = phip_align_one(&b, &o, kind == PHIP_TRACE ? &tr : NULL);
     }
-    int rc = phip_align_one(&b, &o, kind == PHIP_TRACE ? &tr : NULL);
     free(q);
     free(t);
     if (rc != 0) {
         fprintf(stderr, "%s: %s\n", fname, phip_last_error());
         free(tr);
+        free(rcbuf);
+        if (rt) parasail_result_free(rt);
         return NULL;
+    }
+    if (tab == 2) {
+        /* rows hold s2Len entries, columns m (src/memory.c:188-204, 251-275) */
+        const int n = s2Len;
+        int **rows[4], **cols[4];
+        if (kind == PHIP_STATS) {
+            parasail_result_extra_stats_rowcols_t *c = rt->stats->rowcols;
+            rows[0] = &c->score_row; rows[1] = &c->matches_row;
+            rows[2] = &c->similar_row; rows[3] = &c->length_row;
+            cols[0] = &c->score_col; cols[1] = &c->matches_col;
+            cols[2] = &c->similar_col; cols[3] = &c->length_col;
+        } else {
+            rows[0] = &rt->rowcols->score_row;
+            cols[0] = &rt->rowcols->score_col;
+        }
+        for (int k = 0; k < (kind == PHIP_STATS ? 4 : 1); ++k) {
+            memcpy(*rows[k], rcbuf + (size_t)k * (m + n), sizeof(int) * (size_t)n);
+            memcpy(*cols[k], rcbuf + (size_t)k * (m + n) + n, sizeof(int) * (size_t)m);
+        }
+        free(rcbuf);
     }
 
     int w = width;
@@ -215,7 +261,9 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
     }
 
     parasail_result_t *r = NULL;
-    if (kind == PHIP_STATS) {
+    if (rt) {
+        r = rt;
+    } else if (kind == PHIP_STATS) {
         r = phost_result_new_stats();
     } else if (kind == PHIP_TRACE) {
         const int lanes = 64, seglen = (m + lanes - 1) / lanes;
@@ -268,16 +316,26 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
     return r;
 }
 
+parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kind, int width,
+                               int scan, const char *s1, int s1Len, const char *s2, int s2Len,
+                               int open, int gap, const parasail_matrix_t *matrix,
+                               const parasail_profile_t *profile)
+{
+    (void)profile;
+    return align_ex(fname, alg, sg_flags, kind, width, scan, 0, s1, s1Len, s2, s2Len, open, gap,
+                    matrix);
+}
+
 static parasail_result_t *phost_align_profile(const char *fname, int alg, int sg_flags, int kind,
-                                              int width, int scan,
+                                              int width, int scan, int tab,
                                               const parasail_profile_t *profile, const char *s2,
                                               int s2Len, int open, int gap)
 {
     CHECK_NULL(fname, profile);
     CHECK_NULL(fname, profile->matrix);
     CHECK_GT0(fname, profile->s1Len);
-    return phost_align(fname, alg, sg_flags, kind, width, scan, profile->s1, profile->s1Len, s2,
-                       s2Len, open, gap, profile->matrix, profile);
+    return align_ex(fname, alg, sg_flags, kind, width, scan, tab, profile->s1, profile->s1Len, s2,
+                    s2Len, open, gap, profile->matrix);
 }
 
 /* ---- the generated entry points ---- */
@@ -333,7 +391,7 @@ static parasail_result_t *phost_align_profile(const char *fname, int alg, int sg
         const int open, const int gap)                                                       \
     {                                                                                        \
         return phost_align_profile(__func__, ALG_##alg, SGF_##alg, KIND_##kind, WID_##w,     \
-                                   SCAN_##strat, profile, s2, s2Len, open, gap);             \
+                                   SCAN_##strat, 0, profile, s2, s2Len, open, gap);          \
     }
 
 #define DEF_S(alg, kind, strat) \
@@ -344,6 +402,36 @@ static parasail_result_t *phost_align_profile(const char *fname, int alg, int sg
     DEF_S(alg, _stats, scan) DEF_S(alg, _trace, striped) DEF_S(alg, _trace, scan)
 
 PARASAIL_HIP_ALGS(DEF_ALG)
+
+/* the _table and _rowcol forms (KIND "" or _stats, TAB table or rowcol) */
+#define TAB_table 1
+#define TAB_rowcol 2
+#define DEF_TW(alg, kind, tab, strat, w)                                                    \
+    PHOST_EXPORT parasail_result_t *parasail_##alg##kind##_##tab##_##strat##_hip_64_##w(    \
+        const char *const s1, const int s1Len, const char *const s2, const int s2Len,        \
+        const int open, const int gap, const parasail_matrix_t *matrix)                      \
+    {                                                                                        \
+        return align_ex(__func__, ALG_##alg, SGF_##alg, KIND_##kind, WID_##w, SCAN_##strat,  \
+                        TAB_##tab, s1, s1Len, s2, s2Len, open, gap, matrix);                 \
+    }                                                                                        \
+    PHOST_EXPORT parasail_result_t                                                           \
+        *parasail_##alg##kind##_##tab##_##strat##_profile_hip_64_##w(                        \
+            const parasail_profile_t *const profile, const char *const s2, const int s2Len,  \
+            const int open, const int gap)                                                   \
+    {                                                                                        \
+        return phost_align_profile(__func__, ALG_##alg, SGF_##alg, KIND_##kind, WID_##w,     \
+                                   SCAN_##strat, TAB_##tab, profile, s2, s2Len, open, gap);  \
+    }
+#define DEF_TS(alg, kind, tab, strat)                                                       \
+    DEF_TW(alg, kind, tab, strat, 8) DEF_TW(alg, kind, tab, strat, 16)                       \
+    DEF_TW(alg, kind, tab, strat, 32) DEF_TW(alg, kind, tab, strat, sat)
+#define DEF_TALG(alg)                                                                       \
+    DEF_TS(alg, , table, striped) DEF_TS(alg, , table, scan)                                 \
+    DEF_TS(alg, , rowcol, striped) DEF_TS(alg, , rowcol, scan)                               \
+    DEF_TS(alg, _stats, table, striped) DEF_TS(alg, _stats, table, scan)                     \
+    DEF_TS(alg, _stats, rowcol, striped) DEF_TS(alg, _stats, rowcol, scan)
+
+PARASAIL_HIP_ALGS(DEF_TALG)
 
 /* ---- profiles ---- */
 

@@ -13,6 +13,10 @@
 parasail_result_t *phost_result_new(void);
 parasail_result_t *phost_result_new_stats(void);
 parasail_result_t *phost_result_new_trace(int a, int b, size_t alignment, size_t size);
+/* tables (rowcol 0) or last row and column (rowcol 1) of an a x b DP, of
+ * scores alone or (stats) with the three counts; flags TABLE/ROWCOL and
+ * STATS set */
+parasail_result_t *phost_result_new_tables(int a, int b, int stats, int rowcol);
 parasail_profile_t *phost_profile_new(const char *s1, int s1Len, const parasail_matrix_t *matrix);
 void *phost_memalign(size_t alignment, size_t size);
 int phost_flag_lanes(int flag);

@@ -51,13 +51,48 @@
 #define P_ALG(alg) P_S(alg, , striped) P_S(alg, , scan) P_S(alg, _stats, striped) \
     P_S(alg, _stats, scan) P_S(alg, _trace, striped) P_S(alg, _trace, scan)
 
+/* _table / _rowcol entries: kind "" or _stats, tab table (is_table) or
+ * rowcol (is_rowcol) */
+#define IS_TABLE_table 1
+#define IS_TABLE_rowcol 0
+#define IS_ROWCOL_table 0
+#define IS_ROWCOL_rowcol 1
+#define FT_ENTRY(alg, kind, tab, strat, w)                                                  \
+    {parasail_##alg##kind##_##tab##_##strat##_hip_64_##w,                                   \
+     "parasail_" #alg #kind "_" #tab "_" #strat "_hip_64_" #w,                              \
+     ALGLABEL_##kind(alg), #strat, "hip", "64", #w, 64, IS_TABLE_##tab, IS_ROWCOL_##tab, 0,  \
+     STATS_ONLY_##kind, 0},
+#define PT_ENTRY(alg, kind, tab, strat, w, creator)                                         \
+    {parasail_##alg##kind##_##tab##_##strat##_profile_hip_64_##w, creator,                   \
+     "parasail_" #alg #kind "_" #tab "_" #strat "_profile_hip_64_" #w,                      \
+     ALGLABEL_##kind(alg), #strat "_profile", "hip", "64", #w, 64, IS_TABLE_##tab,           \
+     IS_ROWCOL_##tab, 0, STATS_ONLY_##kind, 0},
+#define FT_S(alg, kind, tab, strat)                                                        \
+    FT_ENTRY(alg, kind, tab, strat, 8) FT_ENTRY(alg, kind, tab, strat, 16)                  \
+    FT_ENTRY(alg, kind, tab, strat, 32) FT_ENTRY(alg, kind, tab, strat, sat)
+#define PT_S(alg, kind, tab, strat)                                                        \
+    PT_ENTRY(alg, kind, tab, strat, 8, CREATOR_##kind(8))                                  \
+    PT_ENTRY(alg, kind, tab, strat, 16, CREATOR_##kind(16))                                \
+    PT_ENTRY(alg, kind, tab, strat, 32, CREATOR_##kind(32))                                \
+    PT_ENTRY(alg, kind, tab, strat, sat, CREATOR_##kind(sat))
+#define FT_ALG(alg) FT_S(alg, , table, striped) FT_S(alg, , table, scan)                    \
+    FT_S(alg, , rowcol, striped) FT_S(alg, , rowcol, scan) FT_S(alg, _stats, table, striped) \
+    FT_S(alg, _stats, table, scan) FT_S(alg, _stats, rowcol, striped)                      \
+    FT_S(alg, _stats, rowcol, scan)
+#define PT_ALG(alg) PT_S(alg, , table, striped) PT_S(alg, , table, scan)                    \
+    PT_S(alg, , rowcol, striped) PT_S(alg, , rowcol, scan) PT_S(alg, _stats, table, striped) \
+    PT_S(alg, _stats, table, scan) PT_S(alg, _stats, rowcol, striped)                      \
+    PT_S(alg, _stats, rowcol, scan)
+
 static const parasail_function_info_t functions[] = {
     PARASAIL_HIP_ALGS(F_ALG)
+    PARASAIL_HIP_ALGS(FT_ALG)
     {NULL, NULL, NULL, NULL, NULL, NULL, NULL, 0, 0, 0, 0, 0, 0},
 };
 
 static const parasail_pfunction_info_t pfunctions[] = {
     PARASAIL_HIP_ALGS(P_ALG)
+    PARASAIL_HIP_ALGS(PT_ALG)
     {NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, 0, 0, 0, 0, 0, 0},
 };
 

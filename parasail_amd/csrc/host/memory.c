@@ -86,6 +86,70 @@ parasail_result_t *phost_result_new_trace(int a, int b, size_t alignment, size_t
     return r;
 }
 
+/* parasail_result_new_table1 / rowcol1 / table3 / rowcol3,
+ * src/memory.c:171-275: a (query) by b (target) tables, zeroed; rows hold
+ * b entries, columns a */
+parasail_result_t *phost_result_new_tables(int a, int b, int stats, int rowcol)
+{
+    if (a <= 0 || b <= 0) {
+        fprintf(stderr, "%s: inputs must be > 0\n", __func__);
+        return NULL;
+    }
+    parasail_result_t *r = stats ? phost_result_new_stats() : phost_result_new();
+    if (!r) return NULL;
+    int ok = 1;
+    const size_t ab = (size_t)a * (size_t)b;
+    if (!stats && !rowcol) {
+        r->tables = (parasail_result_extra_tables_t *)calloc(1, sizeof(*r->tables));
+        ok = r->tables && (r->tables->score_table = (int *)calloc(ab, sizeof(int)));
+        if (r->tables) r->flag |= PARASAIL_FLAG_TABLE;
+    } else if (!stats) {
+        r->rowcols = (parasail_result_extra_rowcols_t *)calloc(1, sizeof(*r->rowcols));
+        if (r->rowcols) {
+            r->flag |= PARASAIL_FLAG_ROWCOL;
+            r->rowcols->score_row = (int *)calloc((size_t)b, sizeof(int));
+            r->rowcols->score_col = (int *)calloc((size_t)a, sizeof(int));
+        }
+        ok = r->rowcols && r->rowcols->score_row && r->rowcols->score_col;
+    } else if (!rowcol) {
+        parasail_result_extra_stats_tables_t *t =
+            (parasail_result_extra_stats_tables_t *)calloc(1, sizeof(*t));
+        r->stats->tables = t;
+        r->flag |= PARASAIL_FLAG_STATS;
+        if (t) {
+            r->flag |= PARASAIL_FLAG_TABLE;
+            t->score_table = (int *)calloc(ab, sizeof(int));
+            t->matches_table = (int *)calloc(ab, sizeof(int));
+            t->similar_table = (int *)calloc(ab, sizeof(int));
+            t->length_table = (int *)calloc(ab, sizeof(int));
+        }
+        ok = t && t->score_table && t->matches_table && t->similar_table && t->length_table;
+    } else {
+        parasail_result_extra_stats_rowcols_t *c =
+            (parasail_result_extra_stats_rowcols_t *)calloc(1, sizeof(*c));
+        r->stats->rowcols = c;
+        r->flag |= PARASAIL_FLAG_STATS;
+        if (c) {
+            r->flag |= PARASAIL_FLAG_ROWCOL;
+            c->score_row = (int *)calloc((size_t)b, sizeof(int));
+            c->matches_row = (int *)calloc((size_t)b, sizeof(int));
+            c->similar_row = (int *)calloc((size_t)b, sizeof(int));
+            c->length_row = (int *)calloc((size_t)b, sizeof(int));
+            c->score_col = (int *)calloc((size_t)a, sizeof(int));
+            c->matches_col = (int *)calloc((size_t)a, sizeof(int));
+            c->similar_col = (int *)calloc((size_t)a, sizeof(int));
+            c->length_col = (int *)calloc((size_t)a, sizeof(int));
+        }
+        ok = c && c->score_row && c->matches_row && c->similar_row && c->length_row &&
+             c->score_col && c->matches_col && c->similar_col && c->length_col;
+    }
+    if (!ok) {
+        parasail_result_free(r);
+        return NULL;
+    }
+    return r;
+}
+
 /* parasail_result_free, src/memory.c:277-327 */
 void parasail_result_free(parasail_result_t *result)
 {
@@ -188,6 +252,73 @@ int parasail_result_get_length(const parasail_result_t *const r)
     }
     return r->stats->length;
 }
+/* table and row/column getters, src/memory.c:906-1027 */
+#define NEED(cond, text)                                                            \
+    do {                                                                            \
+        if (!(cond)) {                                                              \
+            fprintf(stderr, "%s: assert(%s) failed\n", __func__, text);            \
+            return NULL;                                                            \
+        }                                                                           \
+    } while (0)
+#define STATS_TABLE(r) (((r)->flag & PARASAIL_FLAG_STATS) && ((r)->flag & PARASAIL_FLAG_TABLE))
+#define STATS_ROWCOL(r) (((r)->flag & PARASAIL_FLAG_STATS) && ((r)->flag & PARASAIL_FLAG_ROWCOL))
+#define PLAIN_TABLE(r) (!((r)->flag & PARASAIL_FLAG_STATS) && ((r)->flag & PARASAIL_FLAG_TABLE))
+#define PLAIN_ROWCOL(r) (!((r)->flag & PARASAIL_FLAG_STATS) && ((r)->flag & PARASAIL_FLAG_ROWCOL))
+
+int *parasail_result_get_score_table(const parasail_result_t *const r)
+{
+    NEED(r, "result");
+    NEED(STATS_TABLE(r) || PLAIN_TABLE(r), "parasail_result_is_table(result)");
+    return STATS_TABLE(r) ? r->stats->tables->score_table : r->tables->score_table;
+}
+#define STATS_TABLE_GETTER(field)                                                   \
+    int *parasail_result_get_##field##_table(const parasail_result_t *const r)      \
+    {                                                                               \
+        NEED(r, "result");                                                          \
+        NEED(STATS_TABLE(r), "parasail_result_is_stats_table(result)");             \
+        return r->stats->tables->field##_table;                                     \
+    }
+STATS_TABLE_GETTER(matches)
+STATS_TABLE_GETTER(similar)
+STATS_TABLE_GETTER(length)
+int *parasail_result_get_score_row(const parasail_result_t *const r)
+{
+    NEED(r, "result");
+    NEED(STATS_ROWCOL(r) || PLAIN_ROWCOL(r), "parasail_result_is_rowcol(result)");
+    return STATS_ROWCOL(r) ? r->stats->rowcols->score_row : r->rowcols->score_row;
+}
+int *parasail_result_get_score_col(const parasail_result_t *const r)
+{
+    NEED(r, "result");
+    NEED(STATS_ROWCOL(r) || PLAIN_ROWCOL(r), "parasail_result_is_rowcol(result)");
+    return STATS_ROWCOL(r) ? r->stats->rowcols->score_col : r->rowcols->score_col;
+}
+#define STATS_ROWCOL_GETTER(field, rc)                                              \
+    int *parasail_result_get_##field##_##rc(const parasail_result_t *const r)       \
+    {                                                                               \
+        NEED(r, "result");                                                          \
+        NEED(STATS_ROWCOL(r), "parasail_result_is_stats_rowcol(result)");           \
+        return r->stats->rowcols->field##_##rc;                                     \
+    }
+STATS_ROWCOL_GETTER(matches, row)
+STATS_ROWCOL_GETTER(similar, row)
+STATS_ROWCOL_GETTER(length, row)
+STATS_ROWCOL_GETTER(matches, col)
+STATS_ROWCOL_GETTER(similar, col)
+STATS_ROWCOL_GETTER(length, col)
+int *parasail_result_get_trace_ins_table(const parasail_result_t *const r)
+{
+    NEED(r, "result");
+    NEED(r->flag & PARASAIL_FLAG_TRACE, "parasail_result_is_trace(result)");
+    return (int *)r->trace->trace_ins_table;
+}
+int *parasail_result_get_trace_del_table(const parasail_result_t *const r)
+{
+    NEED(r, "result");
+    NEED(r->flag & PARASAIL_FLAG_TRACE, "parasail_result_is_trace(result)");
+    return (int *)r->trace->trace_del_table;
+}
+
 int *parasail_result_get_trace_table(const parasail_result_t *const r)
 {
     CHECK_RESULT(r);

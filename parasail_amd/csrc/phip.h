@@ -85,6 +85,14 @@ int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **traces);
  * one download when the pair fits the 64-lane kernel, else phip_align. */
 int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **traces);
 
+/* One pair (npairs == 1, kind score or stats) with its DP tables: tab 1
+ * writes every cell's H (and, for stats, the matches, similar and length of
+ * that H) row-major, qlen*tlen ints into dst[0] (dst[1..3]); tab 2 writes
+ * the last row (tlen ints) followed by the last column (qlen ints).  The
+ * PARASAIL_TABLE / PARASAIL_ROWCOL outputs of src/sw.c:159-170.  0 on
+ * success. */
+int phip_align_tables(const phip_batch_t *b, int tab, phip_out_t *out, int **dst);
+
 /* Traceback of every pair and its CIGAR, both on the device (kind must be
  * PHIP_TRACE).  qraw/traw: the residues as given, upper-cased, at the
  * batch's offsets (the walk compares residues, cigar_template.c:104-118).

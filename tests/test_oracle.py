@@ -655,3 +655,50 @@ def test_banded_brute_force():
         assert exp is not None
         assert got[0] == exp, (s1, s2, o, g, k, got, exp)
         assert got[1:3] == (m - 1, n - 1)
+
+
+# ------------------------------------------------------------ DP tables
+#
+# The _table outputs (src/sw.c:159-170, sg_stats.c:258-263): every cell's H
+# equals the best path into it (brute force), and every cell's statistics
+# the counts along the trace path into that cell.
+
+def test_score_tables_brute_force():
+    rng = random.Random(4000)
+    for alg in (O.SW, O.NW, O.SG):
+        for trial in range(40):
+            mx, alpha = small_matrix(rng)
+            m, n = rng.randint(1, 4), rng.randint(1, 4)
+            s1 = "".join(rng.choice(alpha) for _ in range(m))
+            s2 = "".join(rng.choice(alpha) for _ in range(n))
+            o, g = rng.choice([(1, 1), (3, 1), (5, 2), (2, 0)])
+            flags = rng.choice(SG_FLAGS) if alg == O.SG else (0, 0, 0, 0)
+            H = brute_H(alg, mx.map(s1), mx.map(s2), mx.values(), o, g, flags)
+            for kind in (O.SCORE, O.STATS):
+                res, tabs = O.align_tables(alg, kind, s1, s2, o, g, mx, flags)
+                tab = tabs[0] if kind == O.STATS else tabs
+                assert (tab == H[1:, 1:]).all(), (alg, kind, flags, s1, s2, o, g)
+                assert res == O.align(alg, kind, s1, s2, o, g, mx, flags)[0]
+
+
+def test_stats_tables_are_counts_along_trace_paths():
+    rng = random.Random(4001)
+    b = O.blosum62()
+    dna = O.matrix_create("ACGT", 2, -3)
+    for alg in (O.SW, O.NW, O.SG):
+        for trial in range(12):
+            mx, alpha = (b, "ARNDCQEGHILKMFPSTWYV") if trial % 2 else (dna, "ACGT")
+            s1 = rand_seq(rng, alpha, rng.randint(1, 14))
+            s2 = rand_seq(rng, alpha, rng.randint(1, 14))
+            o, g = rng.choice([(11, 1), (5, 2), (3, 3), (1, 1)])
+            flags = rng.choice(SG_FLAGS) if alg == O.SG else (0, 0, 0, 0)
+            _, tabs = O.align_tables(alg, O.STATS, s1, s2, o, g, mx, flags)
+            _, tr = O.align(alg, O.TRACE, s1, s2, o, g, mx, flags)
+            q, t = mx.map(s1), mx.map(s2)
+            for i in range(len(s1)):
+                for j in range(len(s2)):
+                    # the walk from cell (i, j) (cigar_template.c:143-251)
+                    c = O.cigar(s1, s2, False, i, j, tr)
+                    exp = counts_along_cigar(c[0], c[1], c[2], i, j, q, t, mx.values())
+                    got = (tabs[1][i, j], tabs[2][i, j], tabs[3][i, j])
+                    assert got == exp, (alg, flags, s1, s2, o, g, i, j)
