@@ -292,6 +292,62 @@ int *parasail_result_get_matches_col(const parasail_result_t *const result);
 int *parasail_result_get_similar_col(const parasail_result_t *const result);
 int *parasail_result_get_length_col(const parasail_result_t *const result);
 
+/* alignment strings of a trace result and their printed form
+ * (parasail.h:299-373, src/traceback.c:121-285) */
+typedef struct parasail_traceback_ {
+    char *query;
+    char *comp;
+    char *ref;
+} parasail_traceback_t;
+parasail_traceback_t *parasail_result_get_traceback(parasail_result_t *result, const char *seqA,
+                                                    int lena, const char *seqB, int lenb,
+                                                    const parasail_matrix_t *matrix, char match,
+                                                    char pos, char neg);
+parasail_traceback_t *parasail_result_get_traceback_extra(
+    parasail_result_t *result, const char *seqA, int lena, const char *seqB, int lenb,
+    const parasail_matrix_t *matrix, char match, char pos, char neg, int case_sensitive,
+    const char *alphabet_aliases);
+void parasail_traceback_free(parasail_traceback_t *traceback);
+void parasail_traceback_generic(const char *seqA, int lena, const char *seqB, int lenb,
+                                const char *nameA, const char *nameB,
+                                const parasail_matrix_t *matrix, parasail_result_t *result,
+                                char match, char pos, char neg, int width, int name_width,
+                                int use_stats);
+void parasail_traceback_generic_extra(const char *seqA, int lena, const char *seqB, int lenb,
+                                      const char *nameA, const char *nameB,
+                                      const parasail_matrix_t *matrix, parasail_result_t *result,
+                                      char match, char pos, char neg, int width, int name_width,
+                                      int use_stats, int int_width, FILE *stream);
+void parasail_traceback_generic_extra2(const char *seqA, int lena, const char *seqB, int lenb,
+                                       const char *nameA, const char *nameB,
+                                       const parasail_matrix_t *matrix, parasail_result_t *result,
+                                       char match, char pos, char neg, int width, int name_width,
+                                       int use_stats, int int_width, FILE *stream,
+                                       int case_sensitive, const char *alphabet_aliases);
+
+/* the SSW library's interface emulated on the hip functions
+ * (parasail.h:451-476, src/pssw.c:16-192): best local alignment, its
+ * begin from a pass over the reversed prefixes, its CIGAR from a
+ * traceback of the aligned stretch; 8-bit first, then 16 */
+typedef struct parasail_result_ssw {
+    uint16_t score1;
+    int32_t ref_begin1;
+    int32_t ref_end1;
+    int32_t read_begin1;
+    int32_t read_end1;
+    uint32_t *cigar;
+    int32_t cigarLen;
+} parasail_result_ssw_t;
+parasail_result_ssw_t *parasail_ssw(const char *const s1, const int s1Len, const char *const s2,
+                                    const int s2Len, const int open, const int gap,
+                                    const parasail_matrix_t *matrix);
+parasail_result_ssw_t *parasail_ssw_profile(const parasail_profile_t *const profile,
+                                            const char *const s2, const int s2Len,
+                                            const int open, const int gap);
+parasail_profile_t *parasail_ssw_init(const char *const s1, const int s1Len,
+                                      const parasail_matrix_t *matrix, const int8_t score_size);
+void parasail_result_ssw_free(parasail_result_ssw_t *result);
+
 /* striped or scan table -> row-major int table (src/striped_unwind.c:
  * 5-26, :56-111; parasail/memory.h:137-141): `array` is the result's trace
  * table (or any table of the result's layout) */
