@@ -25,6 +25,7 @@
 
 #include <stdint.h>
 #include <stdio.h>
+#include <sys/types.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -612,6 +613,46 @@ parasail_sequences_t *parasail_sequences_from_file(const char *fname);
 char *parasail_sequences_pack(const parasail_sequences_t *sequences, size_t *size);
 /* replaces parasail_sequences_free (parasail/io.h:59) */
 void parasail_sequences_free(parasail_sequences_t *sequences);
+
+/* The rest of parasail/io.h:18-103 (src/io.c:53-800): a file mapped into
+ * memory, its FASTA/FASTQ statistics and its residues packed with a '$'
+ * after each record, for files and for text already in memory. */
+typedef struct parasail_file {
+    int fd;
+    off_t size;
+    const char *buf;
+} parasail_file_t;
+typedef struct parasail_file_stat {
+    unsigned long sequences;
+    unsigned long characters;
+    unsigned long shortest;
+    unsigned long longest;
+    float mean;
+    float stddev;
+} parasail_file_stat_t;
+parasail_file_t *parasail_open(const char *fname);
+void parasail_close(parasail_file_t *file);
+int parasail_is_fasta(const parasail_file_t *pf);
+int parasail_is_fastq(const parasail_file_t *pf);
+parasail_file_stat_t *parasail_stat(const parasail_file_t *pf);
+parasail_file_stat_t *parasail_stat_fasta(const parasail_file_t *pf);
+parasail_file_stat_t *parasail_stat_fastq(const parasail_file_t *pf);
+char *parasail_read(const parasail_file_t *pf, long *size);
+char *parasail_pack(const parasail_file_t *pf, long *size);
+char *parasail_pack_fasta(const parasail_file_t *pf, long *size);
+char *parasail_pack_fastq(const parasail_file_t *pf, long *size);
+int parasail_is_fasta_buffer(const char *buf, off_t size);
+int parasail_is_fastq_buffer(const char *buf, off_t size);
+parasail_file_stat_t *parasail_stat_buffer(const char *buffer, off_t size);
+parasail_file_stat_t *parasail_stat_fasta_buffer(const char *buffer, off_t size);
+parasail_file_stat_t *parasail_stat_fastq_buffer(const char *buffer, off_t size);
+char *parasail_pack_buffer(const char *buffer, off_t size, long *packed_size);
+char *parasail_pack_fasta_buffer(const char *buffer, off_t size, long *packed_size);
+char *parasail_pack_fastq_buffer(const char *buffer, off_t size, long *packed_size);
+
+/* substitution matrix or PSSM from a text file (src/io.c:921-1178) */
+parasail_matrix_t *parasail_matrix_from_file(const char *filename);
+parasail_matrix_t *parasail_matrix_from_file_case_sensitive(const char *filename);
 
 /* device time (ms, HIP events) of the last call's alignment kernels */
 double parasail_hip_last_kernel_ms(void);

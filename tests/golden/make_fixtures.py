@@ -3,9 +3,10 @@
 Run in a container where the reference tree is mounted; the output
 (tests/golden/fixtures.json) is committed, so nothing at test time reads the
 reference.  Only data is taken: the FASTA inputs the reference's tests use
-(data/*.fasta), the known-answer values of tests/test_basic.c, and the two
-substitution matrices those known answers need (blosum62 and pam50 value
-tables, parsed as numbers from parasail/matrices/*.h).
+(data/*.fasta), the PSSM text files of data/ its matrix reader parses, the
+known-answer values of tests/test_basic.c, and the two substitution matrices
+those known answers need (blosum62 and pam50 value tables, parsed as numbers
+from parasail/matrices/*.h).
 
     python tests/golden/make_fixtures.py /root/reference
 """
@@ -46,12 +47,18 @@ def read_matrix_header(path):
 
 
 def main(ref):
-    out = {"fasta": {}, "matrices": {}, "kat": []}
+    out = {"fasta": {}, "matrices": {}, "kat": [], "matrix_files": {}}
     for fn in ["test_small_2.fasta", "test_P0_2.fasta", "test_P0_3.fasta",
                "test_P0_4.fasta", "test_P0_5.fasta", "test_P0_10.fasta",
                "test_case.fasta", "test_case2.fasta", "test_64.fasta",
                "test_64r.fasta", "test_p29.fasta"]:
         out["fasta"][fn] = read_fasta(os.path.join(ref, "data", fn))
+    # PSSM text files the reference's matrix reader takes (data/*.txt,
+    # read by src/io.c:921-1178 through tests/test_matrix.c), kept as text
+    out["matrix_files"] = {}
+    for fn in ["cd02249_matrix_safe.txt", "cd02249_matrix_noquery.txt"]:
+        with open(os.path.join(ref, "data", fn)) as fh:
+            out["matrix_files"][fn] = fh.read()
     for name in ["blosum62", "pam50"]:
         out["matrices"][name] = read_matrix_header(
             os.path.join(ref, "parasail", "matrices", name + ".h"))

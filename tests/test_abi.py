@@ -199,3 +199,146 @@ def test_sequences_from_file_and_pack(tmp_path):
     L.parasail_sequences_free(p)
     L.parasail_sequences_free(None)
     assert not L.parasail_sequences_from_file(str(tmp_path / "missing.fa").encode())
+
+
+class _FileStat(ctypes.Structure):
+    _fields_ = [("sequences", ctypes.c_ulong), ("characters", ctypes.c_ulong),
+                ("shortest", ctypes.c_ulong), ("longest", ctypes.c_ulong),
+                ("mean", ctypes.c_float), ("stddev", ctypes.c_float)]
+
+
+class _File(ctypes.Structure):
+    _fields_ = [("fd", ctypes.c_int), ("size", ctypes.c_long), ("buf", ctypes.c_void_p)]
+
+
+def _file_api():
+    from parasail_amd import lib
+    L = lib()
+    L.parasail_open.restype = ctypes.POINTER(_File)
+    L.parasail_open.argtypes = [ctypes.c_char_p]
+    L.parasail_close.argtypes = [ctypes.POINTER(_File)]
+    for nm in ("parasail_stat", "parasail_stat_fasta", "parasail_stat_fastq"):
+        getattr(L, nm).restype = ctypes.POINTER(_FileStat)
+        getattr(L, nm).argtypes = [ctypes.POINTER(_File)]
+    for nm in ("parasail_stat_buffer", "parasail_stat_fasta_buffer", "parasail_stat_fastq_buffer"):
+        getattr(L, nm).restype = ctypes.POINTER(_FileStat)
+        getattr(L, nm).argtypes = [ctypes.c_char_p, ctypes.c_long]
+    for nm in ("parasail_read", "parasail_pack", "parasail_pack_fasta", "parasail_pack_fastq"):
+        getattr(L, nm).restype = ctypes.c_void_p
+        getattr(L, nm).argtypes = [ctypes.POINTER(_File), ctypes.POINTER(ctypes.c_long)]
+    L.parasail_pack_buffer.restype = ctypes.c_void_p
+    L.parasail_pack_buffer.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+    L.parasail_is_fasta.argtypes = [ctypes.POINTER(_File)]
+    L.parasail_is_fastq.argtypes = [ctypes.POINTER(_File)]
+    return L
+
+
+def test_file_layer_stat_pack_read(tmp_path):
+    """parasail_open/stat/pack/read/close and their _buffer forms
+    (src/io.c:53-800): FASTA records counted by their letters with '$' after
+    each; FASTQ four-line records, whose statistics sample a record's length
+    when read and again when the next record starts, counting no characters
+    (src/io.c:401-496)."""
+    L = _file_api()
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    fa = tmp_path / "a.fa"
+    text = b">one first\nMKV\nLL\r\n>two\nACDE\n\n>three\nW\n"
+    fa.write_bytes(text)
+    pf = L.parasail_open(str(fa).encode())
+    assert pf and pf.contents.size == len(text)
+    assert L.parasail_is_fasta(pf) == 1 and L.parasail_is_fastq(pf) == 0
+    st = L.parasail_stat(pf).contents
+    assert (st.sequences, st.characters, st.shortest, st.longest) == (3, 10, 1, 5)
+    assert abs(st.mean - 10 / 3) < 1e-5 and abs(st.stddev - 2.0816659) < 1e-5
+    n = ctypes.c_long()
+    p = L.parasail_pack(pf, ctypes.byref(n))
+    assert ctypes.string_at(p, n.value + 1) == b"MKVLL$ACDE$W$\0"
+    libc.free(p)
+    p = L.parasail_read(pf, ctypes.byref(n))
+    assert n.value == len(text) and ctypes.string_at(p, n.value) == text
+    libc.free(p)
+    L.parasail_close(pf)
+    p = L.parasail_pack_buffer(text, len(text), ctypes.byref(n))
+    assert ctypes.string_at(p, n.value) == b"MKVLL$ACDE$W$"
+    libc.free(p)
+    # errors: non-alpha residues, an empty last record, no header
+    assert not L.parasail_stat_fasta_buffer(b">a\nAC1D\n", 8)
+    assert not L.parasail_stat_fasta_buffer(b">a\nACD\n>b\n", 10)
+    assert not L.parasail_stat_fasta_buffer(b"ACD\n", 4)
+    # FASTQ
+    fq = b"@r1\nACGT\n+\nIIII\n@r2\nGG\n+r2\n##\n"
+    st = L.parasail_stat_buffer(fq, len(fq)).contents
+    assert (st.sequences, st.characters, st.shortest, st.longest) == (2, 0, 2, 4)
+    assert abs(st.mean - 10 / 3) < 1e-5  # lengths 4, 4, 2 sampled
+    p = L.parasail_pack_buffer(fq, len(fq), ctypes.byref(n))
+    assert ctypes.string_at(p, n.value) == b"ACGT$GG$"
+    libc.free(p)
+    assert not L.parasail_stat_fastq_buffer(b"@r1\nACGT\nIIII\n", 14)
+    assert not L.parasail_open(str(tmp_path / "missing.fa").encode())
+
+
+def _parse_pssm_text(text):
+    """The values of a PSSM text file, split by whitespace (a plain reading
+    of data/cd02249_matrix_*.txt)."""
+    lines = [ln.split() for ln in text.splitlines() if ln.strip() and not ln.startswith("#")]
+    alphabet = "".join(lines[0])
+    rows, query = [], ""
+    for ln in lines[1:]:
+        if ln[0].lstrip("-").isdigit():
+            query += "*"
+            rows.append([int(v) for v in ln])
+        else:
+            query += ln[0]
+            rows.append([int(v) for v in ln[1:]])
+    return alphabet, query, rows
+
+
+def test_matrix_from_file(tmp_path, fixtures):
+    """parasail_matrix_from_file (src/io.c:921-1178) on the reference's PSSM
+    files (with and without a query letter per row) and on a square matrix
+    written out as text: alphabet, query, values, extrema, type, length and
+    a mapper that sends unknown letters to the last index."""
+    from parasail_amd import lib, Matrix, MP
+    L = lib()
+    L.parasail_matrix_from_file.restype = MP
+    L.parasail_matrix_from_file.argtypes = [ctypes.c_char_p]
+    L.parasail_matrix_from_file_case_sensitive.restype = MP
+    L.parasail_matrix_from_file_case_sensitive.argtypes = [ctypes.c_char_p]
+    L.parasail_matrix_free.argtypes = [MP]
+    for fn, text in fixtures["matrix_files"].items():
+        path = tmp_path / fn
+        path.write_text(text)
+        m = L.parasail_matrix_from_file(str(path).encode())
+        assert m, fn
+        mc = m.contents
+        alphabet, query, rows = _parse_pssm_text(text)
+        assert mc.type == 1 and mc.size == len(alphabet) and mc.length == len(rows)
+        assert mc.alphabet.decode() == alphabet and mc.query.decode() == query
+        vals = [mc.matrix[k] for k in range(mc.size * mc.length)]
+        assert vals == [v for r in rows for v in r]
+        assert (mc.max, mc.min) == (max(vals), min(vals))
+        assert mc.mapper[ord("A")] == alphabet.index("A") == mc.mapper[ord("a")]
+        assert mc.mapper[ord("Z")] == mc.size - 1
+        L.parasail_matrix_free(m)
+    # a square matrix in the same text form (comment, blank line, CRLF)
+    d = fixtures["matrices"]["blosum62"]
+    a = d["alphabet"]
+    body = "# BLOSUM62\n   " + "  ".join(a) + "\r\n\n" + "".join(
+        a[r] + " " + " ".join("%3d" % d["values"][r * 24 + c] for c in range(24)) + "\n"
+        for r in range(24))
+    path = tmp_path / "blosum62.txt"
+    path.write_text(body)
+    for cs in (0, 1):
+        f = L.parasail_matrix_from_file_case_sensitive if cs else L.parasail_matrix_from_file
+        m = f(str(path).encode())
+        assert m
+        mc = m.contents
+        assert mc.type == 0 and mc.size == 24 and mc.length == 24
+        assert [mc.matrix[k] for k in range(576)] == d["values"]
+        assert (mc.max, mc.min) == (d["max"], d["min"])
+        assert mc.mapper[ord("w")] == (23 if cs else a.index("W"))
+        L.parasail_matrix_free(m)
+    # a row short of values, and a row letter outside the alphabet line's form
+    path.write_text("A C\nA 1 2\nC 3\n")
+    assert not L.parasail_matrix_from_file(str(path).encode())

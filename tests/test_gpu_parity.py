@@ -912,3 +912,33 @@ def test_file_to_resident_batch(tmp_path, fixtures, mats):
                     (name, k)
     finally:
         L.parasail_hip_batch_free(h)
+
+
+def test_pssm_from_matrix_file(tmp_path, fixtures):
+    """A PSSM read by parasail_matrix_from_file from the reference's
+    data/cd02249_matrix_safe.txt, aligned per call and batched, equals the
+    oracle with the same position rows (parasail_matrix_pssm_create)."""
+    L = P.lib()
+    L.parasail_matrix_from_file.restype = P.MP
+    L.parasail_matrix_from_file.argtypes = [ctypes.c_char_p]
+    text = fixtures["matrix_files"]["cd02249_matrix_safe.txt"]
+    path = tmp_path / "pssm.txt"
+    path.write_text(text)
+    pm = L.parasail_matrix_from_file(str(path).encode())
+    assert pm and pm.contents.type == 1
+    mc = pm.contents
+    alphabet = mc.alphabet.decode()
+    vals = np.array([mc.matrix[k] for k in range(mc.size * mc.length)], dtype=np.int32)
+    om = O.matrix_pssm_create(alphabet, vals, mc.length)
+    rng = random.Random(41)
+    ts = ["".join(rng.choice(alphabet) for _ in range(rng.randint(20, 300))) for _ in range(24)]
+    for name in ("sw", "nw", "sg", "sg_qb_de"):
+        alg, fl = oracle_alg(name)
+        got = P.align_batch(name, 0, 0, None, ts, 11, 1, pm)
+        for k, t in enumerate(ts):
+            exp, _ = O.align(alg, O.SCORE, None, t, 11, 1, om, fl)
+            assert (got["score"][k], got["end_query"][k], got["end_ref"][k]) == exp[:3], (name, k)
+            if k < 6:
+                one = P.align("parasail_%s_striped_hip_64_sat" % name, None, t, 11, 1, pm)
+                assert (one["score"], one["end_query"], one["end_ref"]) == exp[:3], (name, k)
+    L.parasail_matrix_free(pm)
