@@ -228,7 +228,8 @@ def stats_trace(peak32, npairs=100000):
     substitutions, 0.5% insertions and 0.5% deletions of geometric length,
     mean 2) and its 400 bp window, match 2 mismatch -3, open 5 extend 2;
     semi-global (all four ends free) with statistics, then with traceback
-    and CIGAR strings, int16-first."""
+    and CIGAR strings, in parasail_sg_stats_striped_16 and
+    parasail_sg_trace_striped_16 semantics (width 16)."""
     dm = P.matrix_create("ACGT", 2, -3)
     t0 = time.perf_counter()
     b = WL.config2(npairs=npairs)
@@ -236,14 +237,14 @@ def stats_trace(peak32, npairs=100000):
         % (len(b), b.cells, time.perf_counter() - t0))
     # one untimed call of each at full size first: kernels loaded, and the
     # library's pinned staging buffer grown to the batch (kept for the process)
-    P.align_arrays("sg", 1, 0, b, 5, 2, dm)
-    P.align_arrays("sg", 2, 0, b, 5, 2, dm, want_cigar=True)
-    rs = P.align_arrays("sg", 1, 0, b, 5, 2, dm)
+    P.align_arrays("sg", 1, 16, b, 5, 2, dm)
+    P.align_arrays("sg", 2, 16, b, 5, 2, dm, want_cigar=True)
+    rs = P.align_arrays("sg", 1, 16, b, 5, 2, dm)
     wall_s = P.last_call_ms() / 1e3
     kms_s = P.last_kernel_ms()
     log("stats/trace: stats %.1f ms kernel, %.1f ms call" % (kms_s, wall_s * 1e3))
     t0 = time.perf_counter()
-    rt = P.align_arrays("sg", 2, 0, b, 5, 2, dm, want_cigar=True)
+    rt = P.align_arrays("sg", 2, 16, b, 5, 2, dm, want_cigar=True)
     wall_py = time.perf_counter() - t0
     wall_t = P.last_call_ms() / 1e3
     kms_t = P.last_kernel_ms()

@@ -19,6 +19,26 @@ bool dp32_window_holds(const phip_matrix_t &mx, int alg, int kind, int check, in
 // 64-lane jobs; score-only pairs go whatever the check (extrema computed).
 bool dp32_eligible_one(const phip_matrix_t &mx, int alg, int kind, int check, int open, int gap,
                        int m, int n);
+// Trace codes in the wavefront kernel's layout are 4-bit fields: bits 0-1
+// the source of H (0 none -- SW at 0 --, 1 INS, 2 DEL, 3 DIAG), bit 2 set
+// when E opens (DIAG_E, else INS_E), bit 3 when F opens (DIAG_F, else
+// DEL_F); field ((s*T + t)*rpad(R) + r) of a pair's region holds row
+// t*R + r at step s (column s - t), two fields a byte, low field first.
+__host__ __device__ inline int dp32_rpad(int R) { return (R + 7) & ~7; }
+__host__ __device__ inline int dp32_code_of(int f)
+{
+    return (f & 8 ? 32 : 64) | (f & 4 ? 8 : 16) | ((f & 3) == 3 ? 4 : (f & 3));
+}
+// the reference's trace code (parasail.h:31-41) of cell (i, j) of a pair
+// traced in layout lay (T | R << 8)
+__host__ __device__ inline int dp32_trace_code(const uint8_t *tr, int lay, int64_t i, int64_t j)
+{
+    const int T = lay & 0xff, R = lay >> 8;
+    const int64_t t = i / R, r = i - t * R;
+    const int64_t f = ((j + t) * T + t) * dp32_rpad(R) + r;
+    return dp32_code_of((tr[f >> 1] >> ((f & 1) * 4)) & 15);
+}
+
 // Trace layout code of the wavefront kernel for a query of m rows and jobs
 // of at least min_lanes lanes (T | R << 8; 0 if the pair is outside its
 // range) and the bytes its trace takes for n columns.

@@ -153,7 +153,7 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
             prevRH = up;
             prevRS = upS;
             int hv[R];
-            uint32_t code4 = 0;
+            uint32_t code8 = 0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int Fext = F - gap;
@@ -179,12 +179,14 @@ __device__ __forceinline__ void dp32_run(const Dp32Cfg &c, const Dp32Wave &wd, c
                     dS = dSn;
                 }
                 if (TRACE) {
-                    int code = (cF ? 32 : 64) | (cE ? 8 : 16) | (cD ? 4 : (cFE ? 2 : 1));
-                    if (SW && h == 0) code &= 120;
-                    code4 |= (uint32_t)code << ((r & 3) * 8);
-                    if ((r & 3) == 3) {
-                        *(uint32_t *)(trb + ((int64_t)(s * T + t) * R + (r - 3))) = code4;
-                        code4 = 0;
+                    // the cell's direction as a 4-bit field (dp32.h)
+                    int nib = (cD ? 3 : (cFE ? 2 : 1)) | (cE ? 4 : 0) | (cF ? 8 : 0);
+                    if (SW && h == 0) nib &= 12;  // ZERO_MASK, sw_trace.c:145-147
+                    code8 |= (uint32_t)nib << ((r & 7) * 4);
+                    if ((r & 7) == 7 || r == R - 1) {
+                        constexpr int RP = (R + 7) & ~7;
+                        *(uint32_t *)(trb + (((int64_t)(s * T + t) * RP + (r & ~7)) >> 1)) = code8;
+                        code8 = 0;
                     }
                 }
                 E[r] = En;
@@ -443,7 +445,6 @@ __global__ void __launch_bounds__(256) k_cigar(CigarCfg c)
     const uint8_t *A = c.qraw + c.qoff[p], *B = c.traw + c.toff[p];
     const int8_t *tr = c.trace + c.troff[p];
     const int lay = c.tlay[p];
-    const int LT = lay & 0xff, LR = lay >> 8;
     uint32_t *ops = c.ops + c.opoff[p];
     const int cap = lena + lenb + 2;
     int nops = 0;
@@ -460,8 +461,7 @@ __global__ void __launch_bounds__(256) k_cigar(CigarCfg c)
     };
     auto code_at = [&](int i, int j) -> int {
         if (!lay) return tr[(int64_t)i * lenb + j];
-        const int t = i / LR, r = i - t * LR;
-        return tr[((int64_t)(j + t) * LT + t) * LR + r];
+        return dp32_trace_code((const uint8_t *)tr, lay, i, j);
     };
     const int eq = c.out[p].end_query, er = c.out[p].end_ref;
     int i = eq, j = er;
@@ -638,7 +638,7 @@ int64_t dp32_trace_bytes(int m, int n, int min_lanes)
 {
     int T = 0, R = 0;
     if (!shape(m, T, R, min_lanes)) return (int64_t)m * n;
-    return ((int64_t)n + T + 1) * T * R;
+    return (((int64_t)n + T + 1) * T * dp32_rpad(R) + 1) / 2;
 }
 
 bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t *status,

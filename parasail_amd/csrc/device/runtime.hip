@@ -509,16 +509,13 @@ static int resident_fetch(phip_resident_t *r, phip_out_t *out, int8_t **traces)
                                  hipMemcpyDeviceToHost));
                 continue;
             }
-            // wavefront order (row t*R+r, column j at ((j+t)*T + t)*R + r)
-            // to row-major
-            const int T = lay & 0xff, R = lay >> 8;
+            // wavefront-order 4-bit fields to row-major codes
             wbuf.resize((size_t)dp32_trace_bytes(m, n));
             HIP_OK(hipMemcpy(wbuf.data(), r->dtr + r->troff[i], wbuf.size(), hipMemcpyDeviceToHost));
-            for (int row = 0; row < m; ++row) {
-                const int t = row / R, rr = row - t * R;
+            for (int row = 0; row < m; ++row)
                 for (int j = 0; j < n; ++j)
-                    traces[i][(size_t)row * n + j] = wbuf[((size_t)(j + t) * T + t) * R + rr];
-            }
+                    traces[i][(size_t)row * n + j] =
+                        (int8_t)dp32_trace_code((const uint8_t *)wbuf.data(), lay, row, j);
         }
     }
     return 0;
@@ -654,15 +651,12 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
         }
         memcpy(out, h + o_out, sizeof(phip_out_t));
         if (trbytes) {
-            // wavefront order (row t*R+r, column j at ((j+t)*T + t)*R + r)
-            // to row-major
-            const int lay = dp32_layout(m, 64), T = lay & 0xff, R = lay >> 8;
-            const int8_t *w = (const int8_t *)(h + o_tr);
-            for (int row = 0; row < m; ++row) {
-                const int t = row / R, rr = row - t * R;
+            // wavefront-order 4-bit fields to row-major codes
+            const int lay = dp32_layout(m, 64);
+            const uint8_t *w = (const uint8_t *)(h + o_tr);
+            for (int row = 0; row < m; ++row)
                 for (int j = 0; j < n; ++j)
-                    traces[0][(size_t)row * n + j] = w[((size_t)(j + t) * T + t) * R + rr];
-            }
+                    traces[0][(size_t)row * n + j] = (int8_t)dp32_trace_code(w, lay, row, j);
         }
         rc = 0;
     } while (0);

@@ -213,9 +213,10 @@ def banded_cells(m, n, k):
 def dp32_trace_bytes(m, n):
     """HBM bytes of one pair's trace in the wavefront kernel's layout: T
     lanes (16, 32 or 64, the fewest whose band of <= 20 rows per lane covers
-    the query), R rows per lane (a multiple of 4), (n + T + 1) steps."""
+    the query), R rows per lane (a multiple of 4) taking R rounded up to 8
+    4-bit fields, (n + T + 1) steps."""
     for T in (16, 32, 64):
         if m <= T * 20:
             R = max(4, ((m + T - 1) // T + 3) & ~3)
-            return (n + T + 1) * T * R
+            return ((n + T + 1) * T * ((R + 7) & ~7) + 1) // 2
     return m * n
