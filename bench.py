@@ -19,6 +19,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -103,8 +104,11 @@ class MultiPass:
         return ms
 
 
-def per_call_gcups(reps=1):
-    """Config 0: parasail_sw_striped_profile_hip_64_16, one launch per call."""
+def per_call_gcups(reps=1, threads=1):
+    """Config 0: parasail_sw_striped_profile_hip_64_16 per call, from one
+    host thread (one launch per call) or from `threads` threads over slices
+    of the targets, as tests/test_openmp.c:353-369 runs it under OpenMP
+    (calls made together share a launch)."""
     L = P.lib()
     q, b = WL.config0()
     m = P.blosum62()
@@ -118,13 +122,22 @@ def per_call_gcups(reps=1):
     targets = [b.target(k) for k in range(len(b))]
     for t in targets[:20]:
         L.parasail_result_free(pf(prof, t, len(t), 11, 1))
+    def run(part):
+        for t in part:
+            L.parasail_result_free(pf(prof, t, len(t), 11, 1))
+
     best = None
+    pool = ThreadPoolExecutor(threads) if threads > 1 else None
     for _ in range(reps):
         t0 = time.perf_counter()
-        for t in targets:
-            L.parasail_result_free(pf(prof, t, len(t), 11, 1))
+        if pool is None:
+            run(targets)
+        else:
+            list(pool.map(run, [targets[k::threads] for k in range(threads)]))
         dt = time.perf_counter() - t0
         best = dt if best is None else min(best, dt)
+    if pool is not None:
+        pool.shutdown()
     L.parasail_profile_free(prof)
     return b.cells / best / 1e9, b.cells, best
 
@@ -462,7 +475,10 @@ def main():
     }
     if not a.no_percall:
         g, c0, s0 = per_call_gcups()
+        g16, _, s16 = per_call_gcups(threads=16)
         line["per_pair_kernel"] = {"gcups": round(g, 3), "cells": c0, "seconds": round(s0, 3),
+                                   "gcups_16_threads": round(g16, 3),
+                                   "seconds_16_threads": round(s16, 3),
                                    "function": "parasail_sw_striped_profile_hip_64_16",
                                    "workload": "1 x 300 aa query vs 2,000 targets U[50,1000]"}
         if "sw" in per_alg:

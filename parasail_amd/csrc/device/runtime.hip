@@ -565,9 +565,164 @@ int one_reserve(size_t bytes)
 
 }  // namespace
 
-// One pair through the int32 lane-chain kernel on a whole wave (64 lanes,
-// the shortest chain for one pair): one upload from pinned memory, one
-// launch, one download.  Pairs outside its range take phip_align.
+// Per-call alignments.  One pair runs on a whole wave (64 lanes, the
+// shortest chain for one pair) of the int32 lane-chain kernel.  Calls made
+// at the same time from several host threads -- how parasail's callers use
+// the per-pair functions, OpenMP over pairs (tests/test_openmp.c:353-369)
+// -- are coalesced into one launch: a caller queues its pair, the first
+// caller that finds no launch under way leads, takes every queued pair
+// that shares its scoring (algorithm, kind, end flags, gaps, matrix and
+// saturation check; at most kOneMax) and runs them as one batch of 64-lane
+// jobs with one upload, one launch and one download; the others wait for
+// their results.  A lone caller is a batch of one.  Pairs outside the
+// kernel's range take phip_align.
+namespace {
+
+struct OneReq {
+    const phip_batch_t *b;
+    phip_out_t *out;
+    int8_t **traces;
+    int rc;
+    bool done;
+};
+
+pthread_mutex_t g_oq_lock = PTHREAD_MUTEX_INITIALIZER;
+pthread_cond_t g_oq_cv = PTHREAD_COND_INITIALIZER;
+std::vector<OneReq *> g_oq;
+bool g_oq_busy = false;
+constexpr int kOneMax = 512;
+
+bool same_scoring(const phip_batch_t *a, const phip_batch_t *b)
+{
+    return a->alg == b->alg && a->kind == b->kind && a->sg_flags == b->sg_flags &&
+           a->open == b->open && a->gap == b->gap && a->want_sat_check == b->want_sat_check &&
+           a->matrix.values == b->matrix.values && a->matrix.size == b->matrix.size &&
+           a->matrix.is_pssm == b->matrix.is_pssm;
+}
+
+// one launch for the requests of one scoring; each request's rc is set
+void one_batch(const std::vector<OneReq *> &reqs)
+{
+    const int np = (int)reqs.size();
+    const phip_batch_t *b0 = reqs[0]->b;
+    const int kind = b0->kind, msize = b0->matrix.size;
+    std::vector<int32_t> ql(np), tl(np), status(np, PHIP_ST_W32);
+    std::vector<int64_t> qo(np), to(np), tro(np, 0);
+    int64_t qtot = 0, ttot = 0, trtot = 0;
+    bool ext = false;
+    for (int k = 0; k < np; ++k) {
+        const phip_batch_t *b = reqs[k]->b;
+        ql[k] = b->qlen[0];
+        tl[k] = b->tlen[0];
+        qo[k] = qtot;
+        to[k] = ttot;
+        qtot += ql[k];
+        ttot += tl[k];
+        if (kind == PHIP_TRACE) {
+            tro[k] = trtot;
+            trtot += (dp32_trace_bytes(ql[k], tl[k], 64) + 15) & ~(int64_t)15;
+        }
+        if (!dp32_window_holds(b->matrix, b->alg, kind, b->want_sat_check, b->open, b->gap, ql[k],
+                               tl[k]))
+            ext = true;
+    }
+    HostBatch hb;
+    hb.np = np;
+    hb.qlen = ql.data();
+    hb.tlen = tl.data();
+    hb.qoff = qo.data();
+    hb.toff = to.data();
+    hb.mx = &b0->matrix;
+    std::vector<int> pairs(np);
+    for (int k = 0; k < np; ++k) pairs[k] = k;
+    Dp32Plan P;
+    int rc = -1;
+    if (!dp32_plan(hb, pairs, status.data(), tro.data(), P, 64, kind != PHIP_TRACE)) {
+        for (OneReq *r : reqs) r->rc = phip_align(r->b, r->out, r->traces);
+        return;
+    }
+    {
+        size_t off = 0;
+        auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
+        const size_t o_plan = slot(P.bytes), o_out = slot(sizeof(phip_out_t) * np),
+                     o_qo = slot(8 * np), o_to = slot(8 * np), o_ql = slot(4 * np),
+                     o_tl = slot(4 * np), o_mat = slot((size_t)msize * msize * 4),
+                     o_q = slot((size_t)qtot), o_t = slot((size_t)ttot);
+        const size_t up_end = off;
+        const size_t o_tr = slot((size_t)trtot);
+        pthread_mutex_lock(&g_lock);
+        do {
+            if (one_reserve(off) != 0) break;
+            char *h = g_one_h, *d = g_one_d;
+            memcpy(h + o_plan, P.image.data(), P.bytes);
+            memcpy(h + o_qo, qo.data(), 8 * np);
+            memcpy(h + o_to, to.data(), 8 * np);
+            memcpy(h + o_ql, ql.data(), 4 * np);
+            memcpy(h + o_tl, tl.data(), 4 * np);
+            memcpy(h + o_mat, b0->matrix.values, (size_t)msize * msize * 4);
+            for (int k = 0; k < np; ++k) {
+                memcpy(h + o_q + qo[k], reqs[k]->b->q, (size_t)ql[k]);
+                memcpy(h + o_t + to[k], reqs[k]->b->t, (size_t)tl[k]);
+            }
+            if (hipMemcpyAsync(d, h, up_end, hipMemcpyHostToDevice, g_stream) != hipSuccess) {
+                phip_set_error("phip_align_one: upload failed");
+                break;
+            }
+            DPParams dp;
+            memset(&dp, 0, sizeof(dp));
+            dp.alg = b0->alg;
+            dp.kind = kind;
+            dp.sg_flags = b0->alg == PHIP_SG ? b0->sg_flags : 0;
+            dp.open = b0->open;
+            dp.gap = b0->gap;
+            dp.check = ext ? b0->want_sat_check : 0;
+            dp.matrix = (const int *)(d + o_mat);
+            dp.msize = msize;
+            dp.q = (const uint8_t *)(d + o_q);
+            dp.qoff = (const int64_t *)(d + o_qo);
+            dp.qlen = (const int32_t *)(d + o_ql);
+            dp.t = (const uint8_t *)(d + o_t);
+            dp.toff = (const int64_t *)(d + o_to);
+            dp.tlen = (const int32_t *)(d + o_tl);
+            dp.npairs = np;
+            dp.out = (phip_out_t *)(d + o_out);
+            dp.trace = (int8_t *)(d + o_tr);
+            if (hipEventRecord(g_ev0, g_stream) != hipSuccess) break;
+            if (dp32_launch(dp, P, d + o_plan, g_stream) != 0) break;
+            if (hipEventRecord(g_ev1, g_stream) != hipSuccess) break;
+            bool ok = hipMemcpyAsync(h + o_out, d + o_out, sizeof(phip_out_t) * np,
+                                     hipMemcpyDeviceToHost, g_stream) == hipSuccess;
+            if (trtot)
+                ok &= hipMemcpyAsync(h + o_tr, d + o_tr, (size_t)trtot, hipMemcpyDeviceToHost,
+                                     g_stream) == hipSuccess;
+            ok &= hipStreamSynchronize(g_stream) == hipSuccess;
+            if (!ok) {
+                phip_set_error("phip_align_one: download failed");
+                break;
+            }
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, g_ev0, g_ev1) == hipSuccess) g_last_ms = g_last_main_ms = ms;
+            for (int k = 0; k < np; ++k) {
+                memcpy(reqs[k]->out, h + o_out + sizeof(phip_out_t) * k, sizeof(phip_out_t));
+                if (kind == PHIP_TRACE) {
+                    // wavefront-order 4-bit fields to row-major codes
+                    const int m = ql[k], n = tl[k], lay = dp32_layout(m, 64);
+                    const uint8_t *w = (const uint8_t *)(h + o_tr + tro[k]);
+                    int8_t *dst = reqs[k]->traces[0];
+                    for (int row = 0; row < m; ++row)
+                        for (int j = 0; j < n; ++j)
+                            dst[(size_t)row * n + j] = (int8_t)dp32_trace_code(w, lay, row, j);
+                }
+            }
+            rc = 0;
+        } while (0);
+        pthread_mutex_unlock(&g_lock);
+    }
+    for (OneReq *r : reqs) r->rc = rc;
+}
+
+}  // namespace
+
 extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **traces)
 {
     if (phip_init() != 0) return -1;
@@ -577,91 +732,33 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
     if (m <= 0 || n <= 0 ||
         !dp32_eligible_one(b->matrix, b->alg, b->kind, b->want_sat_check, b->open, b->gap, m, n))
         return phip_align(b, out, traces);
-    const int64_t zero = 0;
-    HostBatch hb;
-    hb.np = 1;
-    hb.qlen = &m;
-    hb.tlen = &n;
-    hb.qoff = &zero;
-    hb.toff = &zero;
-    hb.mx = &b->matrix;
-    const std::vector<int> pairs(1, 0);
-    const int32_t status = PHIP_ST_W32;
-    Dp32Plan P;
-    if (!dp32_plan(hb, pairs, &status, &zero, P, 64, b->kind != PHIP_TRACE))
-        return phip_align(b, out, traces);
-    const int msize = b->matrix.size;
-    size_t off = 0;
-    auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
-    const size_t o_plan = slot(P.bytes), o_out = slot(sizeof(phip_out_t)), o_qo = slot(8),
-                 o_to = slot(8), o_ql = slot(4), o_tl = slot(4),
-                 o_mat = slot((size_t)msize * msize * 4), o_q = slot((size_t)m),
-                 o_t = slot((size_t)n);
-    const size_t up_end = off;
-    const int64_t trbytes = b->kind == PHIP_TRACE ? dp32_trace_bytes(m, n, 64) : 0;
-    const size_t o_tr = slot((size_t)trbytes);
-    pthread_mutex_lock(&g_lock);
-    int rc = -1;
-    do {
-        if (one_reserve(off) != 0) break;
-        char *h = g_one_h, *d = g_one_d;
-        memcpy(h + o_plan, P.image.data(), P.bytes);
-        memset(h + o_qo, 0, 8);
-        memset(h + o_to, 0, 8);
-        memcpy(h + o_ql, &m, 4);
-        memcpy(h + o_tl, &n, 4);
-        memcpy(h + o_mat, b->matrix.values, (size_t)msize * msize * 4);
-        memcpy(h + o_q, b->q, (size_t)m);
-        memcpy(h + o_t, b->t, (size_t)n);
-        if (hipMemcpyAsync(d, h, up_end, hipMemcpyHostToDevice, g_stream) != hipSuccess) {
-            phip_set_error("phip_align_one: upload failed");
-            break;
+    OneReq me{b, out, traces, -1, false};
+    pthread_mutex_lock(&g_oq_lock);
+    g_oq.push_back(&me);
+    while (!me.done) {
+        if (g_oq_busy) {
+            pthread_cond_wait(&g_oq_cv, &g_oq_lock);
+            continue;
         }
-        DPParams dp;
-        memset(&dp, 0, sizeof(dp));
-        dp.alg = b->alg;
-        dp.kind = b->kind;
-        dp.sg_flags = b->alg == PHIP_SG ? b->sg_flags : 0;
-        dp.open = b->open;
-        dp.gap = b->gap;
-        dp.check = dp32_window_holds(b->matrix, b->alg, b->kind, b->want_sat_check, b->open,
-                                     b->gap, m, n)
-                       ? 0 : b->want_sat_check;
-        dp.matrix = (const int *)(d + o_mat);
-        dp.msize = msize;
-        dp.q = (const uint8_t *)(d + o_q);
-        dp.qoff = (const int64_t *)(d + o_qo);
-        dp.qlen = (const int32_t *)(d + o_ql);
-        dp.t = (const uint8_t *)(d + o_t);
-        dp.toff = (const int64_t *)(d + o_to);
-        dp.tlen = (const int32_t *)(d + o_tl);
-        dp.npairs = 1;
-        dp.out = (phip_out_t *)(d + o_out);
-        dp.trace = (int8_t *)(d + o_tr);
-        if (dp32_launch(dp, P, d + o_plan, g_stream) != 0) break;
-        bool ok = hipMemcpyAsync(h + o_out, d + o_out, sizeof(phip_out_t), hipMemcpyDeviceToHost,
-                                 g_stream) == hipSuccess;
-        if (trbytes)
-            ok &= hipMemcpyAsync(h + o_tr, d + o_tr, (size_t)trbytes, hipMemcpyDeviceToHost,
-                                 g_stream) == hipSuccess;
-        ok &= hipStreamSynchronize(g_stream) == hipSuccess;
-        if (!ok) {
-            phip_set_error("phip_align_one: download failed");
-            break;
+        // lead: the oldest request's scoring, every queued request that
+        // shares it
+        g_oq_busy = true;
+        std::vector<OneReq *> batch, rest;
+        const phip_batch_t *key = g_oq.front()->b;
+        for (OneReq *r : g_oq) {
+            if ((int)batch.size() < kOneMax && same_scoring(r->b, key)) batch.push_back(r);
+            else rest.push_back(r);
         }
-        memcpy(out, h + o_out, sizeof(phip_out_t));
-        if (trbytes) {
-            // wavefront-order 4-bit fields to row-major codes
-            const int lay = dp32_layout(m, 64);
-            const uint8_t *w = (const uint8_t *)(h + o_tr);
-            for (int row = 0; row < m; ++row)
-                for (int j = 0; j < n; ++j)
-                    traces[0][(size_t)row * n + j] = (int8_t)dp32_trace_code(w, lay, row, j);
-        }
-        rc = 0;
-    } while (0);
-    pthread_mutex_unlock(&g_lock);
-    return rc;
+        g_oq.swap(rest);
+        pthread_mutex_unlock(&g_oq_lock);
+        one_batch(batch);
+        pthread_mutex_lock(&g_oq_lock);
+        for (OneReq *r : batch) r->done = true;
+        g_oq_busy = false;
+        pthread_cond_broadcast(&g_oq_cv);
+    }
+    pthread_mutex_unlock(&g_oq_lock);
+    return me.rc;
 }
 
 extern "C" int phip_align_tables(const phip_batch_t *b, int tab, phip_out_t *out, int **dst)

@@ -117,6 +117,45 @@ def test_per_call_all_algs(name, mats):
             check_one(name, kind, s1, s2, o, g, pm, om, width="sat")
 
 
+def test_per_call_from_many_threads(mats):
+    """Per-call functions called from 16 host threads at once (the OpenMP
+    pattern of tests/test_openmp.c:353-369): calls made together are served
+    by shared launches, grouped by scoring; every result is checked against
+    the oracle.  The tasks mix algorithms, kinds, widths, matrices and gaps
+    so that groups of several scorings are queued together."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = random.Random(77)
+    tasks = []
+    for k in range(480):
+        key = "blosum62" if k % 3 else "dna"
+        alpha = AA if key == "blosum62" else "ACGT"
+        s1 = rseq(rng, rng.randint(1, 400), alpha)
+        s2 = mutate(rng, s1, alpha) if k % 2 else rseq(rng, rng.randint(1, 400), alpha)
+        name = rng.choice(["sw", "sw", "nw", "sg", "sg_qb_de"])
+        kind = rng.choice(["score", "score", "stats", "trace"])
+        width = rng.choice(["sat", "16", "32"])
+        o, g = rng.choice([(11, 1), (10, 1), (5, 2)])
+        tasks.append((name, kind, width, key, s1, s2, o, g))
+
+    def run(part):
+        out = []
+        for name, kind, width, key, s1, s2, o, g in part:
+            suffix = {"score": "", "stats": "_stats", "trace": "_trace"}[kind]
+            fn = "parasail_%s%s_striped_hip_64_%s" % (name, suffix, width)
+            out.append(P.align(fn, s1, s2, o, g, mats[key][0]))
+        return out
+
+    parts = [tasks[k::16] for k in range(16)]
+    with ThreadPoolExecutor(16) as pool:
+        got = list(pool.map(run, parts))
+    for part, res in zip(parts, got):
+        for (name, kind, width, key, s1, s2, o, g), r in zip(part, res):
+            ctx = (name, kind, width, key, s1, s2, o, g)
+            assert r is not None, ctx
+            want = check_one(name, kind, s1, s2, o, g, mats[key][0], mats[key][1], width=width)
+            assert r == want, ctx
+
+
 @pytest.mark.parametrize("width", ["8", "16", "32"])
 def test_fixed_widths_and_saturation(width, mats):
     rng = random.Random(int(width))
