@@ -39,50 +39,17 @@
 #include <algorithm>
 #include <vector>
 
+#include "pk.h"
 #include "wave.h"
-
-typedef short s2v __attribute__((ext_vector_type(2)));
 
 namespace {
 
-__device__ __forceinline__ int pk_add(int a, int b)
-{
-    return __builtin_bit_cast(int, __builtin_elementwise_add_sat(__builtin_bit_cast(s2v, a),
-                                                                __builtin_bit_cast(s2v, b)));
-}
-__device__ __forceinline__ int pk_sub(int a, int b)
-{
-    return __builtin_bit_cast(int, __builtin_elementwise_sub_sat(__builtin_bit_cast(s2v, a),
-                                                                __builtin_bit_cast(s2v, b)));
-}
-__device__ __forceinline__ int pk_max(int a, int b)
-{
-    return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(s2v, a),
-                                                            __builtin_bit_cast(s2v, b)));
-}
-// unsigned saturating subtraction of both halves (clamps at 0)
-__device__ __forceinline__ int pk_subu(int a, int b)
-{
-    typedef unsigned short u2v __attribute__((ext_vector_type(2)));
-    return __builtin_bit_cast(int, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2v, a),
-                                                                __builtin_bit_cast(u2v, b)));
-}
-__device__ __forceinline__ int lo16(int v) { return (int)(short)(v & 0xffff); }
-__device__ __forceinline__ int hi16(int v) { return v >> 16; }
-__host__ __device__ __forceinline__ int pack16(int lo, int hi) { return (lo & 0xffff) | (hi << 16); }
 __host__ __device__ __forceinline__ int clamp16(int v) { return v < -32768 ? -32768 : (v > 32767 ? 32767 : v); }
-// value of lane-1 (lane 0 of the wave gets an unspecified value)
-__device__ __forceinline__ int shr1(int v)
-{
-    return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, false);
-}
 
 }  // namespace
 
-#define PK_NEG ((int)0x80008000)
 // largest rows per lane (the SW snapshot area in LDS is sized for it)
 constexpr int kSnapRows = 20;
-#define W_NEG (-12000)
 
 // One job (two pairs).  Host-built, device-read.
 struct WaveJob {
@@ -139,7 +106,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
     const WaveJob J = c.jobs[wd.job0 + (jslot < wd.njobs ? jslot : 0)];
     const bool live = jslot < wd.njobs;
     const int open = c.open, gap = c.gap;
-    const int openv = pack16(open, open), gapv = pack16(gap, gap);
+    const int openv = pk::pack(open, open), gapv = pk::pack(gap, gap);
     const bool s1_beg = c.alg == PHIP_SG && (c.sg_flags & PHIP_S1_BEG);
     const bool zero_left = SW || s1_beg;  // column 0 of the DP is all zeros
     const int width = wd.width;
@@ -188,14 +155,14 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 Hs[i] = 0;
                 E[i] = 0;
             } else {
-                HL[i] = pack16(left(first_row[0] + i), left(first_row[1] + i));
+                HL[i] = pk::pack(left(first_row[0] + i), left(first_row[1] + i));
                 E[i] = PK_NEG;
             }
         }
         // diagonal of the lane's first row: SW the H of the row above (0 in
         // column 0), NW/SG its H-open
-        int prevRH = SW ? 0 : pack16(left(first_row[0] - 1), left(first_row[1] - 1));
-        if (!SW && b == 0 && t == 0) prevRH = pack16(-open, -open);  // H[0][0] = 0
+        int prevRH = SW ? 0 : pk::pack(left(first_row[0] - 1), left(first_row[1] - 1));
+        if (!SW && b == 0 && t == 0) prevRH = pk::pack(-open, -open);  // H[0][0] = 0
         int botH = SW ? 0 : PK_NEG, botF = SW ? 0 : PK_NEG;
         int best = (int)0xFFFFFFFF;
         // NW/SG: the lane's rows at each half's last column (SW keeps its
@@ -223,8 +190,8 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         auto step = [&](auto fastc, const int sv, uint32_t &cx, int2 &bx) __attribute__((always_inline)) {
             constexpr bool FAST = decltype(fastc)::value;
             const int j = sv - t;
-            const int xH = shr1(botH);  // every lane enabled
-            const int xF = shr1(botF);
+            const int xH = pk::shr1(botH);  // every lane enabled
+            const int xF = pk::shr1(botF);
             if (FAST || (j >= 0 && j < width)) {
                 const int toA = cx & 0xffff, toB = cx >> 16;
                 int W[R];
@@ -232,34 +199,34 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 for (int i = 0; i < R; ++i) {
                     const short wa = *(const short *)(tabb + (qo[i] & 0xffff) + toA);
                     const short wb = *(const short *)(tabb + ((unsigned)qo[i] >> 16) + toB);
-                    W[i] = pack16(wa, wb);
+                    W[i] = pk::pack(wa, wb);
                 }
                 const int rH = t == 0 ? bx.x : xH;
                 const int rF = t == 0 ? bx.y : xF;
-                int t1 = pk_add(prevRH, W[0]);
+                int t1 = pk::add(prevRH, W[0]);
                 prevRH = rH;
                 // SW passes H between lanes and bands, the others H - open
-                int up = SW ? pk_subu(rH, openv) : rH, F = rF;
+                int up = SW ? pk::subu(rH, openv) : rH, F = rF;
                 int cm = 0;
                 // each HL[i] (SW: Hs[i]) is read (E, next row's diagonal)
                 // before it is rewritten, so it stays in its register
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
                     if (SW) {
-                        E[i] = pk_max(pk_subu(E[i], gapv), HL[i]);
-                        const int t1n = i + 1 < R ? pk_add(Hs[i], W[i + 1]) : 0;
-                        F = pk_max(pk_subu(F, gapv), up);
-                        const int h = pk_max(pk_max(t1, E[i]), F);
+                        E[i] = pk::max(pk::subu(E[i], gapv), HL[i]);
+                        const int t1n = i + 1 < R ? pk::add(Hs[i], W[i + 1]) : 0;
+                        F = pk::max(pk::subu(F, gapv), up);
+                        const int h = pk::max(pk::max(t1, E[i]), F);
                         Hs[i] = h;
-                        HL[i] = pk_subu(h, openv);
+                        HL[i] = pk::subu(h, openv);
                         up = HL[i];
                         t1 = t1n;
                     } else {
-                        E[i] = pk_max(pk_sub(E[i], gapv), HL[i]);
-                        const int t1n = i + 1 < R ? pk_add(HL[i], W[i + 1]) : 0;
-                        F = pk_max(pk_sub(F, gapv), up);
-                        const int h = pk_max(pk_max(t1, E[i]), F);
-                        HL[i] = pk_sub(h, openv);
+                        E[i] = pk::max(pk::sub(E[i], gapv), HL[i]);
+                        const int t1n = i + 1 < R ? pk::add(HL[i], W[i + 1]) : 0;
+                        F = pk::max(pk::sub(F, gapv), up);
+                        const int h = pk::max(pk::max(t1, E[i]), F);
+                        HL[i] = pk::sub(h, openv);
                         up = HL[i];
                         t1 = t1n;
                     }
@@ -277,9 +244,9 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
 #pragma unroll
                     for (int w = 1; w < R; w *= 2)
 #pragma unroll
-                        for (int i = 0; i + w < R; i += 2 * w) mx[i] = pk_max(mx[i], mx[i + w]);
+                        for (int i = 0; i + w < R; i += 2 * w) mx[i] = pk::max(mx[i], mx[i + w]);
                     cm = mx[0];
-                    const int nbst = pk_max(best, cm);
+                    const int nbst = pk::max(best, cm);
                     if (nbst != best) {
                         // the half whose maximum rose stores its rows in LDS
                         // (per-thread slots, rows 256 slots apart)
@@ -333,7 +300,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 if (J.pair[h] < 0) continue;
-                const int S = h ? hi16(best) : lo16(best);
+                const int S = h ? pk::hi(best) : pk::lo(best);
                 const int col = h ? bcolB : bcolA;
                 int row = -1;
 #pragma unroll
@@ -354,7 +321,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
                     const int g = first_row[h] + i;
-                    const int v = (h ? hi16(snap[i]) : lo16(snap[i])) + open;
+                    const int v = (h ? pk::hi(snap[i]) : pk::lo(snap[i])) + open;
                     if (g >= 0 && g < J.m[h] - 1 && v > cmx[h]) { cmx[h] = v; crow[h] = g; }
                 }
             }
@@ -499,7 +466,7 @@ __global__ void __launch_bounds__(256) k_sg_reduce(const int4 *cand, const int2 
         const int nmax = two ? imax(r0.n, r1.n) : r0.n;
         for (int j = lane; j < nmax; j += 64) {
             const int x = row[j].x;
-            const int v0 = lo16(x) + open, v1 = hi16(x) + open;
+            const int v0 = pk::lo(x) + open, v1 = pk::hi(x) + open;
             if (j < r0.n && v0 > rm0) { rm0 = v0; rj0 = j; }
             if (two && j < r1.n && v1 > rm1) { rm1 = v1; rj1 = j; }
         }
@@ -522,7 +489,7 @@ __global__ void __launch_bounds__(256) k_sg_reduce(const int4 *cand, const int2 
     const int cm = lane ? cm1 : cm0, ci = lane ? ci1 : ci0;
     const int rm = lane ? rm1 : rm0, rj = lane ? rj1 : rj0;
     const int x = row[r.n - 1].x;
-    const int corner = (lane ? hi16(x) : lo16(x)) + open;
+    const int corner = (lane ? pk::hi(x) : pk::lo(x)) + open;
     const int m = r.m, nn = r.n;
     int score = corner, eq = m - 1, er = nn - 1;
     if (s1e && s2e) {
@@ -866,7 +833,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     for (int j = 0; j < ntop; ++j) {
         // SW: H = 0 and F clamped at 0; NW/SG: H - open and F = -inf
         const int v = sw ? 0 : zero_top ? -dp.open : clamp16(-dp.open - j * dp.gap - dp.open);
-        top[j] = make_int2(pack16(v, v), sw ? 0 : PK_NEG);
+        top[j] = make_int2(pk::pack(v, v), sw ? 0 : PK_NEG);
     }
     // int32-tier scratch: rows of up to maxn+1 columns per lane
     const size_t per_lane = (size_t)(P->maxn + 1) * 2 * sizeof(int);

@@ -228,10 +228,64 @@ def test_profile_functions(mats):
             r = pf(prof, t.encode(), len(t), 11, 1)
             exp, _ = O.align(O.SW, O.SCORE, q, t, 11, 1, om)
             rc = r.contents
-            if not rc.flag & P.FLAG_SATURATED:
+            # SW striped saturates when the score passes 2^w - max - 2 on the
+            # biased scale (src/sw_striped_avx2_256_8.c), and then reports
+            # score 2^w - 1 with zero ends
+            sat = w == "8" and exp[0] > (1 << 8) - 11 - 2
+            assert bool(rc.flag & P.FLAG_SATURATED) == sat, (w, t)
+            if sat:
+                assert (rc.score, rc.end_query, rc.end_ref) == ((1 << 8) - 1, 0, 0)
+            else:
                 assert (rc.score, rc.end_query, rc.end_ref) == exp[:3]
             L.parasail_result_free(r)
         L.parasail_profile_free(prof)
+
+
+def test_config0_all_targets():
+    """The per-pair workload in full: the 300-residue query against all
+    2,000 targets through parasail_sw_striped_profile_hip_64_16 (and its
+    stats form), called from 16 host threads as tests/test_openmp.c:353-369
+    calls the per-pair functions, every result against the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    from parasail_amd import workloads as WL
+    L = P.lib()
+    L.parasail_profile_free.argtypes = [P.ctypes.c_void_p]
+    pm, om = P.blosum62(), O.blosum62()
+    q, b = WL.config0()
+    targets = [b.target(k) for k in range(len(b))]
+    assert len(targets) == 2000
+    for fn in ("parasail_sw_striped_profile_hip_64_16", "parasail_sw_stats_striped_profile_hip_64_16"):
+        create = P.ctypes.CFUNCTYPE(P.ctypes.c_void_p, P.ctypes.c_char_p, P.ctypes.c_int, P.MP)(
+            L.parasail_lookup_pcreator(fn.encode()))
+        pf = P.ctypes.CFUNCTYPE(P.RP, P.ctypes.c_void_p, P.ctypes.c_char_p, P.ctypes.c_int,
+                                P.ctypes.c_int, P.ctypes.c_int)(
+            L.parasail_lookup_pfunction(fn.encode()))
+        prof = create(q, len(q), pm)
+        assert prof
+
+        def run(ks):
+            out = []
+            for k in ks:
+                r = pf(prof, targets[k], len(targets[k]), 11, 1)
+                rc = r.contents
+                row = (rc.score, rc.end_query, rc.end_ref, rc.flag & P.FLAG_SATURATED)
+                if rc.flag & P.FLAG_STATS:
+                    st = P.ctypes.cast(rc.extra, P.ctypes.POINTER(P.StatsExtra)).contents
+                    row += (st.matches, st.similar, st.length)
+                L.parasail_result_free(r)
+                out.append((k, row))
+            return out
+
+        with ThreadPoolExecutor(16) as pool:
+            got = dict(x for part in pool.map(run, [range(k, 2000, 16) for k in range(16)])
+                       for x in part)
+        L.parasail_profile_free(prof)
+        stats = "stats" in fn
+        for k in range(2000):
+            exp, _ = O.align(O.SW, O.STATS if stats else O.SCORE, q.decode(),
+                             targets[k].decode(), 11, 1, om)
+            want = tuple(exp[:3]) + (0,) + (tuple(exp[3:]) if stats else ())
+            assert got[k] == want, (fn, k)
 
 
 def test_pssm(mats):
@@ -341,7 +395,7 @@ def test_batch_stats_trace_dna(mats):
         for col, key in enumerate(("score", "end_query", "end_ref", "matches", "similar", "length")):
             assert (st[key] == exp[:, col]).all(), (name, key)
         tr = P.align_batch(name, 2, 16, qs, ts, 5, 2, pm, want_cigar=True)
-        for k in range(0, len(qs), 7):
+        for k in range(len(qs)):
             e, t = O.align(alg, O.TRACE, qs[k], ts[k], 5, 2, om, flags)
             c = O.cigar(qs[k], ts[k], alg == O.SG, e[1], e[2], t)
             expc = None if c is None else (O.cigar_string(c[0]), c[1], c[2])
@@ -551,6 +605,40 @@ def test_long_nw_int32(mats):
         assert (got["score"][i], got["end_query"][i], got["end_ref"][i]) == tuple(exp[i, :3]), i
     r = P.align("parasail_nw_scan_hip_64_32", qs[1], ts[1], 5, 2, pm)
     assert r["score"] == exp[1, 0]
+
+
+def test_long_nw_config3_extremes(mats):
+    """Long-pair lengths at both ends of the 20,000-200,000 bp range, 13
+    groups of bands for the longest, where the oracle is too slow for the
+    full DP: a copy with one block of k residues deleted or inserted scores
+    exactly 2(n - k') - open - (k - 1) gap under match 2 / open 5 / extend 2
+    (at most min(m, n) matched columns of at most 2 each, at least k
+    residues in gaps, one gap the cheapest way to leave them), which the
+    plain alignment reaches.  Banded NW with k = 256 on the same pairs is
+    compared with the oracle's banded DP (200,000 x 513 cells), and the
+    20,000 bp mutated pair with the oracle's full DP."""
+    pm, om = mats["dna"]
+    prng = random.Random(31)
+    q1 = rseq(prng, 200000, "ACGT")
+    t1 = q1[:123457] + q1[123457 + 77:]             # 77 deleted
+    q2 = rseq(prng, 150000, "ACGT")
+    t2 = q2[:40000] + rseq(prng, 33, "ACGT") + q2[40000:]  # 33 inserted
+    q3 = rseq(prng, 20000, "ACGT")
+    t3 = mutate(prng, q3, "ACGT", sub=0.1, indel=0.02)
+    qs, ts = [q1, q2, q3], [t1, t2, t3]
+    got = P.align_batch("nw", 0, 32, qs, ts, 5, 2, pm)
+    want = [2 * 199923 - 5 - 76 * 2, 2 * 150000 - 5 - 32 * 2]
+    for i in range(2):
+        assert (got["score"][i], got["end_query"][i], got["end_ref"][i]) == \
+            (want[i], len(qs[i]) - 1, len(ts[i]) - 1), i
+    exp, _ = O.batch(O.NW, O.SCORE, [om.map(q3)], [om.map(t3)], 5, 2, om)
+    assert (got["score"][2], got["end_query"][2], got["end_ref"][2]) == tuple(exp[0, :3])
+    band = P.nw_banded_batch(qs, ts, 5, 2, 256, pm)
+    for i in range(3):
+        e = O.nw_banded(qs[i], ts[i], 5, 2, 256, om)
+        assert (band["score"][i], band["end_query"][i], band["end_ref"][i]) == e[:3], i
+    # the block's offset (77, 33) lies inside the band: banded equals global
+    assert band["score"][0] == want[0] and band["score"][1] == want[1]
 
 
 def test_long_nw_int32_table_scores(mats):
