@@ -34,6 +34,18 @@ __device__ __forceinline__ int subu(int a, int b)
 __device__ __forceinline__ int lo(int v) { return (int)(short)(v & 0xffff); }
 __device__ __forceinline__ int hi(int v) { return v >> 16; }
 __host__ __device__ __forceinline__ int pack(int l, int h) { return (l & 0xffff) | (h << 16); }
+// maximum of three for non-negative halves below 0x7C00: read as binary16
+// such a half is +0, a subnormal or a finite positive number, and those
+// order as their bit patterns do, so v_pk_maximum3_f16 gives the integer
+// maximum of each half (a half at or above 0x7C00 -- +inf or NaN -- yields
+// a half at or above 0x7C00)
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int max3u15(int a, int b, int c)
+{
+    const h2v x = __builtin_bit_cast(h2v, a), y = __builtin_bit_cast(h2v, b),
+              z = __builtin_bit_cast(h2v, c);
+    return __builtin_bit_cast(int, __builtin_elementwise_maximum(__builtin_elementwise_maximum(x, y), z));
+}
 // value of lane-1 within the wave (lane 0 gets an unspecified value)
 __device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, false); }
 

@@ -73,7 +73,10 @@ struct WaveCfg {
     const WaveJob *jobs;
     const WaveDesc *waves;
     int nwaves;
-    const uint32_t *colcode;  // per job column stream: (toffA | toffB << 16)
+    const uint2 *colcode;     // per job column stream: x = the dwords holding
+                              // the two residues' table entries (toffA |
+                              // toffB << 16), y = the v_perm selector taking
+                              // each entry's half
     const short *tab;         // bank-replicated table of W+open (see k_wave)
     int tabn;                 // table size in dwords
     int qstride, ncopy;       // bytes per query-code row; copies (32 or 1)
@@ -175,35 +178,42 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
 
         // column codes: ccl[k] is the code pair of column k - t; the stream
         // holds T leading and 8 trailing pad codes around the wave's width
-        const uint32_t *ccl = c.colcode + J.ccoff + (T - t);
+        const uint2 *ccl = c.colcode + J.ccoff + (T - t);
         // row above the band, indexed by step (lane 0 reads column == step)
         const int2 *bbs = (b == 0 || !band_live) ? c.toprow
                                                 : c.bandbuf + J.bboff + (int64_t)(b - 1) * wd.bbw;
         const bool wr_lane = t == T - 1 && band_live && (!SW || b < J.nb - 1);
         int2 *const bbr = c.bandbuf + J.bboff + (int64_t)b * wd.bbw;
 
-        // one DP column; the code and the row-above entry used here were
-        // fetched two steps earlier into the same registers, and the next
-        // ones are fetched after their last use, so no value is copied and
-        // each wait is for a load issued more than a step before
+        // the rows' scores against one column: W + open (SW: W) of both
+        // halves from the LDS table, at the code pair's byte offsets
+        int W[R];
+        // (the dword holding each half's entry is read whole: a 32-bit load
+        // result needs no masking wherever the compiler places the select)
+        auto lookup = [&](const int i, const uint2 code) __attribute__((always_inline)) {
+            const unsigned wa = *(const unsigned *)(tabb + (qo[i] & 0xffff) + (code.x & 0xffff));
+            const unsigned wb = *(const unsigned *)(tabb + ((unsigned)qo[i] >> 16) + (code.x >> 16));
+            W[i] = (int)__builtin_amdgcn_perm(wb, wa, code.y);
+        };
+
+        // one DP column.  W holds this column's scores, looked up during
+        // the previous step: each row's entry for the next column is read
+        // from LDS as soon as this column has used it, so the reads are in
+        // flight while the rest of the column is computed.  cn holds the
+        // next column's code pair, fetched two steps earlier, and is
+        // refilled three columns ahead after its last use; bx, the row
+        // above for lane 0, is fetched two steps ahead the same way.
         // FAST: every lane's column lies in [0, width) (the steady state)
-        auto step = [&](auto fastc, const int sv, uint32_t &cx, int2 &bx) __attribute__((always_inline)) {
+        auto step = [&](auto fastc, const int sv, uint2 &cn, int2 &bx) __attribute__((always_inline)) {
             constexpr bool FAST = decltype(fastc)::value;
             const int j = sv - t;
             const int xH = pk::shr1(botH);  // every lane enabled
             const int xF = pk::shr1(botF);
             if (FAST || (j >= 0 && j < width)) {
-                const int toA = cx & 0xffff, toB = cx >> 16;
-                int W[R];
-#pragma unroll
-                for (int i = 0; i < R; ++i) {
-                    const short wa = *(const short *)(tabb + (qo[i] & 0xffff) + toA);
-                    const short wb = *(const short *)(tabb + ((unsigned)qo[i] >> 16) + toB);
-                    W[i] = pk::pack(wa, wb);
-                }
                 const int rH = t == 0 ? bx.x : xH;
                 const int rF = t == 0 ? bx.y : xF;
                 int t1 = pk::add(prevRH, W[0]);
+                lookup(0, cn);
                 prevRH = rH;
                 // SW passes H between lanes and bands, the others H - open
                 int up = SW ? pk::subu(rH, openv) : rH, F = rF;
@@ -215,6 +225,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                     if (SW) {
                         E[i] = pk::max(pk::subu(E[i], gapv), HL[i]);
                         const int t1n = i + 1 < R ? pk::add(Hs[i], W[i + 1]) : 0;
+                        if (i + 1 < R) lookup(i + 1, cn);
                         F = pk::max(pk::subu(F, gapv), up);
                         const int h = pk::max(pk::max(t1, E[i]), F);
                         Hs[i] = h;
@@ -224,6 +235,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                     } else {
                         E[i] = pk::max(pk::sub(E[i], gapv), HL[i]);
                         const int t1n = i + 1 < R ? pk::add(HL[i], W[i + 1]) : 0;
+                        if (i + 1 < R) lookup(i + 1, cn);
                         F = pk::max(pk::sub(F, gapv), up);
                         const int h = pk::max(pk::max(t1, E[i]), F);
                         HL[i] = pk::sub(h, openv);
@@ -235,16 +247,27 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 botF = F;
                 if (wr_lane) bbr[j] = make_int2(botH, botF);
                 if (SW) {
-                    // the rows' maximum as a balanced tree (independent
-                    // maxima issue back to back; a serial chain would wait
-                    // on every result)
+                    // the rows' maximum as a tree of three-way maxima
+                    // (independent maxima issue back to back; a serial chain
+                    // would wait on every result).  Every H is >= 0, and
+                    // below 0x7C00 unless the pair passes the overflow limit
+                    // (then the maximum is at or above 0x7C00 as well)
                     int mx[R];
 #pragma unroll
                     for (int i = 0; i < R; ++i) mx[i] = Hs[SW ? i : 0];
+                    int nm = R;
 #pragma unroll
-                    for (int w = 1; w < R; w *= 2)
+                    for (int lvl = 0; lvl < 5; ++lvl) {
+                        if (nm == 1) break;
+                        int k = 0;
 #pragma unroll
-                        for (int i = 0; i + w < R; i += 2 * w) mx[i] = pk::max(mx[i], mx[i + w]);
+                        for (int i = 0; i < nm; i += 3) {
+                            if (i + 2 < nm) mx[k++] = pk::max3u15(mx[i], mx[i + 1], mx[i + 2]);
+                            else if (i + 1 < nm) mx[k++] = pk::max(mx[i], mx[i + 1]);
+                            else mx[k++] = mx[i];
+                        }
+                        nm = k;
+                    }
                     cm = mx[0];
                     const int nbst = pk::max(best, cm);
                     if (nbst != best) {
@@ -272,11 +295,20 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                         for (int i = 0; i < R; ++i) snap[i] = (HL[i] & mask) | (snap[i] & ~mask);
                     }
                 }
+            } else {
+#pragma unroll
+                for (int i = 0; i < R; ++i) lookup(i, cn);
             }
-            cx = ccl[sv + 2];
+            cn = ccl[sv + 3];
             bx = bbs[sv + 2];
         };
-        uint32_t c0 = ccl[0], c1 = ccl[1];
+        // column codes: c1 holds odd columns' and c0 even columns' (the
+        // step at sv looks up column sv + 1)
+        uint2 c0 = ccl[0];
+#pragma unroll
+        for (int i = 0; i < R; ++i) lookup(i, c0);
+        c0 = ccl[2];
+        uint2 c1 = ccl[1];
         int2 b0 = bbs[0], b1 = bbs[1];
         const std::integral_constant<bool, true> fastc;
         const std::integral_constant<bool, false> slowc;
@@ -285,16 +317,16 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         const int lo = min(T, steps), hi = max(lo, width - 1);
         int s = 0;
         for (; s < lo; s += 2) {
-            step(slowc, s, c0, b0);
-            step(slowc, s + 1, c1, b1);
+            step(slowc, s, c1, b0);
+            step(slowc, s + 1, c0, b1);
         }
         for (; s < hi; s += 2) {
-            step(fastc, s, c0, b0);
-            step(fastc, s + 1, c1, b1);
+            step(fastc, s, c1, b0);
+            step(fastc, s + 1, c0, b1);
         }
         for (; s < steps; s += 2) {
-            step(slowc, s, c0, b0);
-            step(slowc, s + 1, c1, b1);
+            step(slowc, s, c1, b0);
+            step(slowc, s + 1, c0, b1);
         }
         if (SW && band_live) {
 #pragma unroll
@@ -309,6 +341,10 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                     const int v = (int)snp[(h * kSnapRows + i) * 256];
                     if (g >= 0 && g < J.m[h] && v == S) row = g;
                 }
+                // a maximum at or above 0x7C00 (past the overflow limit,
+                // where the rows' maximum may differ from every row) still
+                // reaches the reduction, which sends the pair to the int32 tier
+                if (row < 0 && S >= 0x7C00) row = 0;
                 if (row >= 0 && (S > cS[h] || (S == cS[h] && col < cC[h]))) {
                     cS[h] = S; cC[h] = col; cR[h] = row;
                 }
@@ -515,7 +551,7 @@ __global__ void __launch_bounds__(256) k_sg_reduce(const int4 *cand, const int2 
 // pairs' columns).
 __global__ void k_colcode(const WaveJob *jobs, const int *clen, const int *cT, int njobs,
                           const uint8_t *t, const int64_t *toff, int PADT, int ncopy,
-                          uint32_t *colcode)
+                          uint2 *colcode)
 {
     const int jb = blockIdx.x;
     if (jb >= njobs) return;
@@ -527,9 +563,12 @@ __global__ void k_colcode(const WaveJob *jobs, const int *clen, const int *cT, i
         const int j = k - T;
         uint32_t a = (ta && j >= 0 && j < J.n[0]) ? ta[j] : PADT;
         uint32_t bb = (tb && j >= 0 && j < J.n[1]) ? tb[j] : PADT;
-        const uint32_t oa = (a >> 1) * ncopy * 4 + (a & 1) * 2;
-        const uint32_t ob = (bb >> 1) * ncopy * 4 + (bb & 1) * 2;
-        colcode[J.ccoff + k] = oa | (ob << 16);
+        // the table holds two columns' entries per dword (even column in
+        // the low half); the selector takes bytes 0-1 or 2-3 of A's dword
+        // into the low half and of B's into the high half
+        const uint32_t oa = (a >> 1) * ncopy * 4, ob = (bb >> 1) * ncopy * 4;
+        const uint32_t sel = 0x05040100u + (a & 1) * 0x0202u + (bb & 1) * 0x02020000u;
+        colcode[J.ccoff + k] = make_uint2(oa | (ob << 16), sel);
     }
 }
 
@@ -614,7 +653,7 @@ struct WavePlan {
     WaveJob *djobs = nullptr;
     WaveDesc *dwaves = nullptr;
     short *dtab = nullptr;
-    uint32_t *dcc = nullptr;
+    uint2 *dcc = nullptr;
     int2 *dbb = nullptr, *dtop = nullptr;
     int4 *dcd = nullptr;
     PairRef *drefs = nullptr;
@@ -715,7 +754,9 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
                 }
             }
     }
-    P->ovf_limit = 32767 - (mx.max + dp.open + 1);
+    // SW: every H of a pair below the limit stays below 0x7C00 (the rows'
+    // maximum is taken on binary16 patterns); NW/SG: below int16's bound
+    P->ovf_limit = (sw ? 0x7BFF : 32767) - (mx.max + dp.open + 1);
     // jobs: consecutive pairs of the same class and band count share one
     // job (A low, B high); then entries of 64/T jobs
     std::vector<WaveJob> jobs;
@@ -843,7 +884,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     size_t off = 0;
     auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
     size_t o_jobs = slot(nj * sizeof(WaveJob)), o_waves = slot(qw.size() * sizeof(WaveDesc)),
-           o_tab = slot(htab.size() * 2), o_cc = slot((size_t)ccoff * 4),
+           o_tab = slot(htab.size() * 2), o_cc = slot((size_t)ccoff * 8),
            o_clen = slot(nj * 4), o_cT = slot(nj * 4), o_bb = slot((size_t)bboff * 8),
            o_top = slot((size_t)ntop * 8),
            o_cd = slot((size_t)cdoff * 16), o_refs = slot(refs.size() * sizeof(PairRef)),
@@ -855,7 +896,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->djobs = (WaveJob *)(b + o_jobs);
     P->dwaves = (WaveDesc *)(b + o_waves);
     P->dtab = (short *)(b + o_tab);
-    P->dcc = (uint32_t *)(b + o_cc);
+    P->dcc = (uint2 *)(b + o_cc);
     P->dbb = (int2 *)(b + o_bb);
     P->dtop = (int2 *)(b + o_top);
     P->dcd = (int4 *)(b + o_cd);

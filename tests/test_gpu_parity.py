@@ -906,8 +906,9 @@ def _concat(seqs):
 def test_batch_int16_overflow_redo(mats):
     """Config-1-shaped batches with pairs the packed-int16 kernel cannot
     hold.  SW: with a match score of 120, self pairs of 300 and 700 residues
-    score past the int16 overflow limit 32767 - (max + open + 1) and are
-    redone by the int32 tier after the run-time test of
+    score past the kernel's overflow limit 0x7BFF - (max + open + 1) (its
+    rows' maximum is taken on binary16 bit patterns, exact below 0x7C00) and
+    are redone by the int32 tier after the run-time test of
     src/sw_striped_avx2_256_16.c:350-362; the 300 one fits the biased 16-bit
     window (2^16 - max - 2), the 700 one only 32 bits, and width 16 reports
     it saturated (src/sw_striped_avx2_256_16.c:374-382).  NW/SG: a pair past
@@ -937,6 +938,10 @@ def test_batch_int16_overflow_redo(mats):
         if name == "sw":
             qn[40], tn[40] = s300, s300
             qn[170], tn[170] = s700, s700
+            # just below the limit (int16), between the limit and int16's
+            # own bound, and at that bound
+            for k, ln in ((80, 263), (120, 265), (200, 272)):
+                qn[k], tn[k] = s300[:ln], s300[:ln]
         qn[260], tn[260] = lng, mutate(rng, lng, AA, 0.1, 0.02)
         qcat, qoff, qlen = _concat(qn)
         tcat, toff, tlen = _concat(tn)
@@ -960,8 +965,10 @@ def test_batch_int16_overflow_redo(mats):
             assert got["flag"][260] & P.FLAG_BITS_32
             continue
         bits = P.FLAG_BITS_8 | P.FLAG_BITS_16 | P.FLAG_BITS_32
-        ovf = 32767 - (120 + 11 + 1)
+        ovf = 0x7BFF - (120 + 11 + 1)
         assert ovf < got["score"][40] <= 65535 - 122 < got["score"][170]
+        assert got["score"][80] == 263 * 120 <= ovf < 265 * 120 == got["score"][120]
+        assert got["score"][200] == 272 * 120 > 32767 - (120 + 11 + 1)
         for k in (40, 170, 260):
             one = P.align("parasail_sw_striped_hip_64_sat", qn[k], tn[k], 11, gap, pm)
             assert int(got["flag"][k]) & bits == one["flag"] & bits, k
