@@ -510,6 +510,16 @@ int parasail_hip_nw_banded_batch(const char *s1, const int64_t *s1_off, const in
                                  int npairs, int open, int gap, int k,
                                  const parasail_matrix_t *matrix,
                                  parasail_hip_batch_result_t *out);
+/* Tuning of the banded entry points for long pairs (no counterpart in the
+ * reference): a pair whose target is long enough is cut into segments of
+ * about `columns` target columns that run at the same time and are joined
+ * exactly (a pair whose join fails is aligned again whole).  0 turns
+ * segments off, a negative value keeps the setting; returns the previous
+ * value (default 8192). */
+int parasail_hip_nw_banded_set_segment(int columns);
+/* of the last banded call: pairs cut into segments, segments, and pairs
+ * aligned again whole because their segments did not join */
+void parasail_hip_nw_banded_last_stats(int *pairs, int *segments, int *redone);
 
 /* A batch kept resident in device memory (HBM): the residues are mapped and
  * uploaded once, and every parasail_hip_batch_align call over it launches

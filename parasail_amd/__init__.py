@@ -459,3 +459,20 @@ def _nw_banded(qcat, qo, ql, tcat, to, tl, open_, gap, k, matrix):
         raise RuntimeError("parasail_hip_nw_banded_batch failed: %s"
                            % lib().parasail_hip_last_error().decode(errors="replace"))
     return res
+
+
+def nw_banded_set_segment(columns):
+    """Columns a segment of a long banded pair owns (0: no segments);
+    returns the previous setting."""
+    L = lib()
+    L.parasail_hip_nw_banded_set_segment.restype = ctypes.c_int
+    L.parasail_hip_nw_banded_set_segment.argtypes = [ctypes.c_int]
+    return L.parasail_hip_nw_banded_set_segment(int(columns))
+
+
+def nw_banded_last_stats():
+    """(segmented pairs, segments, pairs aligned again whole) of the last
+    banded call."""
+    v = [ctypes.c_int(0) for _ in range(3)]
+    lib().parasail_hip_nw_banded_last_stats(*[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)

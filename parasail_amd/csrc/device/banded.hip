@@ -28,6 +28,16 @@ namespace {
 
 __device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, false); }
 __device__ __forceinline__ int shl1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, false); }
+// v of lane t-1; lane 0 gets old
+__device__ __forceinline__ int shr1_or(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xF, 0xF, false);
+}
+// v of lane t+1; lane 63 gets old
+__device__ __forceinline__ int shl1_or(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xF, 0xF, false);
+}
 
 }  // namespace
 
@@ -272,6 +282,373 @@ __global__ void __launch_bounds__(512) k_banded_chain(BandCfg c)
 #undef BAND_R
 }
 
+// ------------------------------------------------------------ segments
+//
+// Long pairs in the diagonal coordinates above, cut along the target into
+// segments that run at the same time, one 64-lane chain each -- the method
+// of Maleki, Musuvathi and Mytkowicz, "Parallelizing dynamic programming
+// through rank convergence" (PPoPP 2014).  Between two columns the band's
+// state (H and E of every slot; F restarts at -inf in every column) maps to
+// the next column's by a max-plus map: every new value is a max of old
+// values plus constants, so adding d to every real value of a column adds d
+// to every real value after it, and the -inf slots stay -inf.  A segment
+// other than the first starts X columns before its own first column, from
+// 0 in every band slot; once it has gone far enough its state differs from
+// the true one by a single constant in every real slot.  The join checks
+// that exactly, at the column before each segment's own first column: the
+// previous segment's state there and this segment's must differ by one
+// constant d in every real slot, with -inf (<= -inf/2) in the same slots.
+// Then this segment's values are the true ones minus the sum of the d's up
+// to it, and the pair's score is the last segment's plus that sum.  A pair
+// whose check fails anywhere is aligned again without segments.
+//
+// A segment's run covers steps [c0, c0 + len): lane t works on column
+// s - t, as in band_chain.  At its first step the lanes' columns lie before
+// c0; from the arbitrary start they just compute more of the warm-up.
+// Every lane stores its slots' (H, E) at the column cs (the state the join
+// compares with the previous segment's end) and at the column ce (the
+// state the next segment is compared with).
+struct SegDesc {
+    int32_t pair, R;
+    int32_t c0;    // first step; 0 for a pair's first segment (the reference's start)
+    int32_t len;   // steps of the run (even)
+    int32_t cs;    // column of the stored start state, -1 for the first segment
+    int32_t ce;    // column of the stored end state, -1 for the last segment
+    int64_t voff;  // int2 offset of the states: start at voff, end at voff + 64 R
+};
+
+struct SegPair {
+    int32_t pair, first, count, colLen;
+};
+
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, N - 1>)
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+__device__ __forceinline__ int med3(int a, int b, int c) { return max(min(a, b), min(max(a, b), c)); }
+
+// The square matrix is staged in LDS (tab).  RP: tab also holds, after the
+// matrix, each query letter's profile -- W + open of every target letter in
+// 4-bit fields -- and the steps where every band slot is a row of the query
+// and every lane a column of the target run R at a time with it:
+//  * H is kept as H - open, so E = max(Hl - open, El - gap) and the same for
+//    F take two instructions each, and the diagonal term is H - open plus
+//    the profile's W + open;
+//  * a slot's row moves one down the query per step, so slot r at step
+//    s + u of a block of R steps reads its profile from qp[(r + u) % R]: the
+//    register slot 0 has just used takes the profile of the row entering
+//    slot R-1 at the next step, and no register is copied;
+//  * the slots past the band's end are held at -inf by H = med3(max(E, F),
+//    H_diag, cap) with cap = -inf there (their H_diag stays within the
+//    host's score bound of -inf, so the median stays <= -inf/2) and
+//    cap = INT32_MAX elsewhere, where the median is the max.
+template <int R, bool RP>
+__device__ __forceinline__ void seg_chain(const BandCfg &c, const int *tab, const SegDesc &d, int2 *vec,
+                                          int *raw, const int si, const int t)
+{
+    const int pr = d.pair;
+    const int m = c.qlen[pr], n = c.tlen[pr];
+    const uint8_t *s1 = c.q + c.qoff[pr];
+    const uint8_t *s2 = c.t + c.toff[pr];
+    const int open = c.open, gap = c.gap, k = c.k, msize = c.msize;
+    const int low = (m > n) ? n - m - k : m - n - k;
+    const int colLen = k - low + 1;
+    const int colOff0 = (m > n) ? -k : low;
+    const int tR = t * R;
+    const int s0 = d.c0;
+    const bool first = d.cs < 0;
+
+    int Hb[R], Eb[R], qm[R], cap[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = tR + r;
+        int h = 0, e = BNEG;
+        if (first) {
+            // the reference's initial arrays (src/nw_banded.c:110-138)
+            e = 0;
+            if (i >= -colOff0 + 1) { h = -open - (i - (-colOff0 + 1)) * gap; e = BNEG; }
+            if (i == -colOff0 - 1) { h = -open; e = BNEG; }
+            if (i == -colOff0) { h = 0; e = BNEG; }
+        }
+        if (i >= colLen) { h = BNEG; e = BNEG; }
+        Hb[r] = h;
+        Eb[r] = e;
+        cap[r] = i < colLen ? INT32_MAX : BNEG;
+        const int pos = colOff0 + s0 - t + i;  // the slot's row at the lane's first column
+        qm[r] = (int)s1[pos < 0 ? 0 : (pos >= m ? m - 1 : pos)] * msize;
+    }
+    auto ldq = [&](int pos) -> int { return (int)s1[pos < 0 ? 0 : (pos >= m ? m - 1 : pos)]; };
+    auto ldt = [&](int j) -> int { return (int)s2[j < 0 ? 0 : (j >= n ? n - 1 : j)]; };
+    int botH = BNEG, botF = BNEG;
+    int tq0 = ldt(s0 - t), tq1 = ldt(s0 + 1 - t);
+    int nq0 = ldq(colOff0 + s0 - t + tR + R), nq1 = ldq(colOff0 + s0 + 1 - t + tR + R);
+    int2 *const vst = vec + d.voff + tR, *const ven = vec + d.voff + 64 * R + tR;
+
+    auto step = [&](auto fastc, const int s, const bool chk, int &tq, int &nq) __attribute__((always_inline)) {
+        constexpr bool FAST = decltype(fastc)::value;
+        const int j = s - t;
+        const bool act = j >= 0 && j < n;
+        // slot -1 of lane 0 is the band's -inf edge
+        int Hup = shr1_or(BNEG, botH), F = shr1_or(BNEG, botF);
+        const int colOff = colOff0 + j;
+        if (!FAST && first && colOff0 + s - 63 < 0) {
+            // the slot just above row 0 holds the column's top boundary
+            const int ib = -colOff - 1 - tR;
+            const int bval = -open - j * gap;
+            const bool bst = act && colOff < 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) Hb[r] = (bst && r == ib) ? bval : Hb[r];
+        }
+        const int tc = tq;
+        int W[R];
+        bool val[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int pos = colOff + tR + r;
+            val[r] = act & (pos >= 0) & (pos < m) & (tR + r < colLen);
+            W[r] = tab[qm[r] + tc];
+        }
+        auto slot = [&](int r, int Hn, int En) __attribute__((always_inline)) {
+            const int Hdag = Hb[r] + W[r];
+            const int E = max(Hn - open, En - gap);
+            const int Fn = max(Hup - open, F - gap);
+            if (FAST) {
+                Eb[r] = E;
+                F = Fn;
+                Hb[r] = med3(max(E, Fn), Hdag, cap[r]);
+            } else {
+                const bool v = val[r];
+                Eb[r] = v ? E : Eb[r];
+                F = v ? Fn : F;
+                Hb[r] = v ? max(max(E, Fn), Hdag) : Hb[r];
+            }
+            Hup = Hb[r];
+        };
+        slot(0, Hb[1], Eb[1]);
+        // lane t+1's first slot, just computed for column j-1, is the left
+        // neighbour of this lane's last slot; lane 63's is the -inf edge
+        const int xHn = shl1_or(BNEG, Hb[0]), xEn = shl1_or(BNEG, Eb[0]);
+#pragma unroll
+        for (int r = 1; r < R; ++r) slot(r, r + 1 < R ? Hb[r + 1] : xHn, r + 1 < R ? Eb[r + 1] : xEn);
+        botH = Hup;
+        botF = F;
+        if (chk) {
+            if (j == d.cs) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) vst[r] = make_int2(Hb[r], Eb[r]);
+            }
+            if (j == d.ce) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) ven[r] = make_int2(Hb[r], Eb[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R - 1; ++r) qm[r] = qm[r + 1];
+        qm[R - 1] = nq * msize;
+        tq = ldt(j + 2);
+        nq = ldq(colOff + 2 + tR + R);
+    };
+    const std::integral_constant<bool, false> slow;
+    const std::integral_constant<bool, true> fast;
+    // steps whose every lane is on a column of the target and every band
+    // slot on a row of the query (as in band_chain)
+    const int tl = (colLen - 1) / R;
+    const int sa = (max(63, -colOff0) + 1) & ~1;
+    const int sb = max(sa, min(n, m + tl + 1 - colOff0 - colLen) & ~1);
+    const int s1e = s0 + d.len;
+    // the stores fall in the 64 steps from cs and from ce
+    auto chk_at = [&](int s) -> bool {
+        return (unsigned)(s + 1 - d.cs) < 65u || (unsigned)(s + 1 - d.ce) < 65u;
+    };
+    int s = s0;
+    for (const int e = min(sa, s1e); s < e; s += 2) {
+        const bool chk = chk_at(s);
+        step(slow, s, chk, tq0, nq0);
+        step(slow, s + 1, chk, tq1, nq1);
+    }
+    if (RP) {
+        // blocks of R fast steps; the rest of the fast steps run as slow
+        // steps below, which compute the same values there
+        const int e = s + (min(sb, s1e) - s) / R * R;
+        if (s < e) {
+            const int *prof = tab + ((msize * msize + 3) & ~3);
+            int qp[R];
+            // H - open from here on, the bottom row handed down included
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                Hb[r] -= open;
+                qp[r] = prof[ldq(colOff0 + s - t + tR + r)];
+            }
+            botH -= open;
+            // the row entering slot R-1 after step s' is colOff0 + s' - t +
+            // tR + R; its letter and the step's target letter are loaded
+            // two steps ahead
+            int ta = ldt(s - t), tb = ldt(s + 1 - t);
+            int na = ldq(colOff0 + s - t + tR + R), nb = ldq(colOff0 + s + 1 - t + tR + R);
+            for (; s < e; s += R) {
+                static_for<R>([&](auto uc) __attribute__((always_inline)) {
+                    constexpr int U = decltype(uc)::value;
+                    const int sp = s + U;
+                    int &tq = (U & 1) ? tb : ta;
+                    int &nq = (U & 1) ? nb : na;
+                    const int tc4 = tq << 2;
+                    int Hup = shr1_or(BNEG, botH), F = shr1_or(BNEG, botF);
+                    auto slot = [&](int r, int HLn, int En) __attribute__((always_inline)) {
+                        const int Wp = (int)__builtin_amdgcn_ubfe((unsigned)qp[(r + U) % R], tc4, 4);
+                        const int Hdag = Hb[r] + Wp;
+                        const int E = max(HLn, En - gap);
+                        F = max(Hup, F - gap);
+                        Eb[r] = E;
+                        Hb[r] = med3(max(E, F), Hdag, cap[r]) - open;
+                        Hup = Hb[r];
+                    };
+                    slot(0, Hb[1], Eb[1]);
+                    qp[U % R] = prof[nq];
+                    const int xHn = shl1_or(BNEG, Hb[0]), xEn = shl1_or(BNEG, Eb[0]);
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        slot(r, r + 1 < R ? Hb[r + 1] : xHn, r + 1 < R ? Eb[r + 1] : xEn);
+                    botH = Hup;
+                    botF = F;
+                    if (chk_at(sp & ~1)) {
+                        const int j = sp - t;
+                        if (j == d.cs) {
+#pragma unroll
+                            for (int r = 0; r < R; ++r) vst[r] = make_int2(Hb[r] + open, Eb[r]);
+                        }
+                        if (j == d.ce) {
+#pragma unroll
+                            for (int r = 0; r < R; ++r) ven[r] = make_int2(Hb[r] + open, Eb[r]);
+                        }
+                    }
+                    tq = ldt(sp + 2 - t);
+                    nq = ldq(colOff0 + sp + 2 - t + tR + R);
+                });
+            }
+            // back to the slow step's state at step s
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                Hb[r] += open;
+                qm[r] = ldq(colOff0 + s - t + tR + r) * msize;
+            }
+            botH += open;
+            tq0 = ldt(s - t);
+            tq1 = ldt(s + 1 - t);
+            nq0 = ldq(colOff0 + s - t + tR + R);
+            nq1 = ldq(colOff0 + s + 1 - t + tR + R);
+        }
+    } else {
+        for (const int e = min(sb, s1e); s < e; s += 2) {
+            const bool chk = chk_at(s);
+            step(fast, s, chk, tq0, nq0);
+            step(fast, s + 1, chk, tq1, nq1);
+        }
+    }
+    for (; s < s1e; s += 2) {
+        const bool chk = chk_at(s);
+        step(slow, s, chk, tq0, nq0);
+        step(slow, s + 1, chk, tq1, nq1);
+    }
+    if (d.ce < 0) {
+        // the last segment: the score as band_chain reads it, relative to
+        // this segment's start
+        const int sidx = (m > n) ? -low : k;
+        int score = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) score = (tR + r == sidx) ? Hb[r] : score;
+        if (sidx >= tR && sidx < tR + R) raw[si] = score;
+    }
+}
+
+// One wave per segment, 4 segments per workgroup; slots per lane from the
+// descriptor (at most RMAX).  LDS: the square matrix, then (RP) each query
+// letter's 4-bit profile.
+template <int RMAX, bool RP>
+__global__ void __launch_bounds__(256) k_banded_seg(BandCfg c, const SegDesc *segs, const int32_t *order,
+                                                    int nseg, int2 *vec, int *raw)
+{
+    extern __shared__ int tab[];
+    const int ms = c.msize;
+    for (int x = threadIdx.x; x < ms * ms; x += blockDim.x) tab[x] = c.mat[x];
+    if (RP) {
+        int *prof = tab + ((ms * ms + 3) & ~3);
+        for (int x = threadIdx.x; x < ms; x += blockDim.x) {
+            int pf = 0;
+            for (int y = 0; y < ms; ++y) pf |= (c.mat[x * ms + y] + c.open) << (4 * y);
+            prof[x] = pf;
+        }
+    }
+    __syncthreads();
+    const int li = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (li >= nseg) return;
+    const int t = threadIdx.x & 63;
+    const int si = __builtin_amdgcn_readfirstlane(order[li]);
+    const SegDesc d = segs[si];
+#define SEG_R(RR) seg_chain<(RMAX >= RR ? RR : 8), RP>(c, tab, d, vec, raw, si, t)
+    switch (__builtin_amdgcn_readfirstlane(d.R)) {
+    case 8: SEG_R(8); break;
+    case 10: if (RMAX >= 10) SEG_R(10); break;
+    case 12: if (RMAX >= 12) SEG_R(12); break;
+    case 14: if (RMAX >= 14) SEG_R(14); break;
+    case 16: if (RMAX >= 16) SEG_R(16); break;
+    case 20: if (RMAX >= 20) SEG_R(20); break;
+    case 24: if (RMAX >= 24) SEG_R(24); break;
+    case 32: if (RMAX >= 32) SEG_R(32); break;
+    default: break;
+    }
+#undef SEG_R
+}
+
+// One wave per segmented pair: the join described above.  A pair whose
+// states do not line up gets status PHIP_ST_ERR (aligned again by the host).
+__global__ void __launch_bounds__(64) k_seg_join(BandCfg c, const SegPair *sp, const SegDesc *segs,
+                                                 const int2 *vec, const int *raw)
+{
+    const SegPair p = sp[blockIdx.x];
+    const int t = threadIdx.x;
+    const int thr = BNEG / 2;
+    int sum = 0;
+    bool ok = true;
+    for (int q = 1; q < p.count && ok; ++q) {
+        const SegDesc a = segs[p.first + q - 1], b = segs[p.first + q];
+        const int2 *ea = vec + a.voff + 64 * a.R;
+        const int2 *sb = vec + b.voff;
+        int lo = INT32_MAX, hi = INT32_MIN, bad = 0;
+        for (int x = t; x < p.colLen; x += 64) {
+            const int2 u = ea[x], v = sb[x];
+            const bool uh = u.x > thr, vh = v.x > thr, ue = u.y > thr, ve = v.y > thr;
+            bad |= (uh != vh) | (ue != ve);
+            if (uh && vh) { lo = min(lo, u.x - v.x); hi = max(hi, u.x - v.x); }
+            if (ue && ve) { lo = min(lo, u.y - v.y); hi = max(hi, u.y - v.y); }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o));
+            hi = max(hi, __shfl_xor(hi, o));
+            bad |= __shfl_xor(bad, o);
+        }
+        ok = !bad && lo == hi && lo != INT32_MAX;
+        sum += lo;
+    }
+    if (t == 0) {
+        const int m = c.qlen[p.pair], n = c.tlen[p.pair];
+        phip_out_t o;
+        memset(&o, 0, sizeof(o));
+        o.score = ok ? raw[p.first + p.count - 1] + sum : 0;
+        o.end_query = m - 1;
+        o.end_ref = n - 1;
+        o.status = ok ? PHIP_ST_W32 : PHIP_ST_ERR;
+        c.out[p.pair] = o;
+    }
+}
+
 // Single-lane fallback for bands wider than 64 x 32 slots (one pair per
 // block): the reference loop over a band kept in HBM scratch.
 __global__ void __launch_bounds__(64) k_banded_one(BandCfg c, int *scratch, int64_t stride)
@@ -366,12 +743,6 @@ __host__ __device__ __forceinline__ int rows_spacing(int br, int send, int nwv)
 }
 
 __device__ __forceinline__ int rol1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, false); }
-// v of lane t-1; lane 0 gets old
-__device__ __forceinline__ int shr1_or(int old, int v)
-{
-    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xF, 0xF, false);
-}
-
 template <int R, bool RP>
 __device__ __forceinline__ void band_rows(const BandCfg &c, const int *tab, const int pr, const int w,
                                           const int t, const int NWV, int2 *rings)
@@ -642,6 +1013,63 @@ BFn chain_kernel(int rmax, bool ltab)
 
 const int kBandRs[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
 
+// k_banded_seg: slots per lane from this set (even: the profile rotation
+// keeps a step's two prefetch registers on the same parity)
+const int kSegRs[] = {8, 10, 12, 14, 16, 20, 24, 32};
+// segment columns: 0 no segments, 1 sized to fill the device once, more:
+// about that many columns a segment
+int g_seg_cols = 1;
+int g_seg_stats[3];  // last batch: pairs on k_banded_seg, segments, pairs redone without
+// warm-up columns before a segment's own: three band widths plus 256.  On
+// long mutated DNA pairs (10% substitutions, 2% indels, k = 256) the states
+// of 3,169 segment starts lined up after 0.8 band widths on average and
+// after 2.6 at most.
+int64_t seg_warmup(int64_t colLen) { return (3 * colLen + 256 + 63) & ~(int64_t)63; }
+
+struct SegCand {
+    int pair, R;
+    int64_t n, colLen, X, Lmin;
+};
+// instruction issues of one step of a one-wave chain with R slots a lane
+int64_t seg_step_cost(int R) { return 9 * R + 12; }
+
+// Segments per candidate pair.  Fixed columns: about cols own columns a
+// segment.  Automatic: the least run cost D whose segments number at most
+// waves (one round of the device's resident waves), every segment's run
+// (warm-up, own columns, 64 steps of lane fill) costing at most D.
+std::vector<int64_t> seg_plan(const std::vector<SegCand> &cs, int cols, int64_t waves)
+{
+    std::vector<int64_t> S(cs.size(), 1);
+    auto count = [&](const SegCand &c, int64_t D) -> int64_t {
+        const int64_t cost = seg_step_cost(c.R);
+        if ((c.n + 64) * cost <= D) return 1;
+        const int64_t L = (D / cost - c.X - 64) & ~(int64_t)1;
+        if (L < c.Lmin) return std::max<int64_t>(1, (c.n - c.X) / c.Lmin);  // the finest split
+        return (c.n - c.X + L - 1) / L;
+    };
+    if (cols >= 2) {
+        for (size_t x = 0; x < cs.size(); ++x)
+            S[x] = std::max<int64_t>(1, (cs[x].n - cs[x].X + cols / 2) / cols);
+    } else {
+        int64_t lo = 1, hi = 1;
+        for (const SegCand &c : cs) hi = std::max(hi, (c.n + 64) * seg_step_cost(c.R));
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            int64_t tot = 0;
+            for (const SegCand &c : cs) tot += count(c, mid);
+            if (tot <= waves) hi = mid;
+            else lo = mid + 1;
+        }
+        for (size_t x = 0; x < cs.size(); ++x) S[x] = count(cs[x], lo);
+    }
+    // own columns of at least Lmin each, or one segment
+    for (size_t x = 0; x < cs.size(); ++x) {
+        if (S[x] >= 2) S[x] = std::min<int64_t>(S[x], (cs[x].n - cs[x].X) / cs[x].Lmin);
+        if (S[x] < 2) S[x] = 1;
+    }
+    return S;
+}
+
 // k_banded_rows: four waves per pair, rows per lane from this set
 constexpr int kRowsWaves = 4;
 const int kRowsRs[] = {1, 2, 3, 4, 6, 8};
@@ -688,16 +1116,10 @@ size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
         break;                                                                    \
     }
 
-extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *out)
+static int banded_impl(const phip_batch_t *b, int k, phip_out_t *out, bool allow_seg)
 {
-    if (phip_init() != 0) return -1;
     const int np = b->npairs;
     if (np <= 0) return 0;
-    // s1 is required even with a PSSM, as in the reference (src/nw_banded.c:50)
-    if (k <= 0 || b->open < 0 || b->gap < 0 || !b->q || !b->t) {
-        phip_set_error("phip_nw_banded_batch: bad arguments");
-        return -1;
-    }
     // lane chains of 64 x nwv lanes (nwv waves of a workgroup per pair:
     // more for few pairs, which otherwise leave most SIMDs idle) for bands
     // up to 64 x nwv x 32 slots, the single-lane loop for wider ones; gaps
@@ -721,6 +1143,13 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         if (v + b->open < 0 || v + b->open > 15) rows_rp = false;
     }
     const bool rows_ok = rows_rp || (!is_pssm && msz <= 64);
+    // segments (k_banded_seg): square matrices staged in LDS
+    const bool seg_ok = !is_pssm && msz <= 64;
+    std::vector<SegCand> cands;
+    std::vector<SegDesc> sdesc;
+    std::vector<SegPair> spairs;
+    int64_t vtot = 0;
+    int seg_rmax = 0;
     for (int i = 0; i < np; ++i) {
         const int m = b->qlen[i], n = b->tlen[i];
         if (m <= 0 || n <= 0) {
@@ -732,6 +1161,25 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         const int64_t low = (m > n) ? (int64_t)n - m - k : (int64_t)m - n - k;
         const int64_t colLen = k - low + 1;
         maxcol = std::max(maxcol, colLen);
+        const bool bounded = ((int64_t)m + n + 2) * ((int64_t)b->open + b->gap + wabs) < (1 << 27);
+        // one-wave lane chains over segments (k_banded_seg) for bands of
+        // 384 to 2,048 slots (8 to 32 per lane, at least 3/4 of them filled)
+        if (allow_seg && seg_ok && bounded && colLen >= 384 && colLen <= 64 * 32 && g_seg_cols > 0) {
+            SegCand sc;
+            sc.pair = i;
+            sc.n = n;
+            sc.colLen = colLen;
+            sc.R = 0;
+            for (int r : kSegRs)
+                if (colLen <= 64 * r) { sc.R = r; break; }
+            sc.X = seg_warmup(colLen);
+            const int64_t colOff0 = (m > n) ? -(int64_t)k : low;
+            // a segment's own columns: at least its warm-up, and every step
+            // of a later segment's run a fast one
+            sc.Lmin = std::max(sc.X, (std::max<int64_t>(63, -colOff0) + 1) & ~(int64_t)1);
+            cands.push_back(sc);
+            continue;
+        }
         if (rows_ok && m >= 512 && m <= 1 << 24 && colLen <= 1 << 14 &&
             ((int64_t)m + n + 2) * ((int64_t)b->open + b->gap + wabs) < (1 << 27)) {
             const int R = rows_pick(colLen, m);
@@ -758,6 +1206,40 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
             rmax = std::max(rmax, R);
         }
     }
+    if (!cands.empty()) {
+        // segments of each candidate pair: the first from the reference's
+        // start, the others from X warm-up columns before their own L
+        // columns (the last owns the rest), each storing the states the join
+        // compares
+        int cus = 0, clk = 0;
+        if (phip_device_info(&cus, &clk) != 0 || cus <= 0) cus = 256;
+        const std::vector<int64_t> S = seg_plan(cands, g_seg_cols, (int64_t)cus * 8);
+        for (size_t x = 0; x < cands.size(); ++x) {
+            const SegCand &sc = cands[x];
+            const int64_t n = sc.n, X = sc.X, Sx = S[x];
+            const int64_t L = Sx >= 2 ? ((n - X) / Sx) & ~(int64_t)1 : 0;
+            SegPair p;
+            p.pair = sc.pair;
+            p.first = (int)sdesc.size();
+            p.count = (int)Sx;
+            p.colLen = (int)sc.colLen;
+            spairs.push_back(p);
+            for (int64_t q = 0; q < Sx; ++q) {
+                const int64_t J = q == 0 ? 0 : X + q * L, Jn = q + 1 == Sx ? n : X + (q + 1) * L;
+                SegDesc d;
+                d.pair = sc.pair;
+                d.R = sc.R;
+                d.c0 = (int)(q == 0 ? 0 : J - X);
+                d.len = (int)(q + 1 == Sx ? ((n + 64 - d.c0) & ~(int64_t)1) : Jn + 64 - d.c0);
+                d.cs = (int)(q == 0 ? -1 : J - 1);
+                d.ce = (int)(q + 1 == Sx ? -1 : Jn - 1);
+                d.voff = vtot;
+                vtot += 2 * 64 * sc.R;
+                sdesc.push_back(d);
+            }
+            seg_rmax = std::max(seg_rmax, sc.R);
+        }
+    }
     // one launch each; the longest chains start first
     std::stable_sort(chain.begin(), chain.end(), [&](int x, int y) { return work[x] > work[y]; });
     std::stable_sort(rows.begin(), rows.end(), [&](int x, int y) { return work[x] > work[y]; });
@@ -770,6 +1252,17 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
                  o_tl = slot((size_t)np * 4), o_pairs = slot((size_t)np * 4), o_rsel = slot((size_t)np * 4),
                  o_out = slot((size_t)np * sizeof(phip_out_t)),
                  o_scr = slot(wide.empty() ? 0 : (size_t)wide.size() * 2 * stride * 4);
+    const size_t nseg = sdesc.size();
+    const size_t o_sd = slot(nseg * sizeof(SegDesc)), o_so = slot(nseg * 4),
+                 o_sp = slot(spairs.size() * sizeof(SegPair)), o_vec = slot((size_t)vtot * sizeof(int2)),
+                 o_raw = slot(nseg * 4);
+    // a pair's segments are consecutive descriptors (the join walks them in
+    // order); they launch costliest first, 4 to a workgroup, through sord
+    std::vector<int32_t> sord(nseg);
+    for (size_t x = 0; x < nseg; ++x) sord[x] = (int32_t)x;
+    std::stable_sort(sord.begin(), sord.end(), [&](int32_t x, int32_t y) {
+        return (int64_t)sdesc[x].len * (sdesc[x].R + 6) > (int64_t)sdesc[y].len * (sdesc[y].R + 6);
+    });
     phip_lock();
     char *base = (char *)dev_alloc(off);
     if (!base) {
@@ -813,6 +1306,13 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
         ok &= hipMemcpyAsync(dpairs, order.data(), order.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
         if (!rs.empty())
             ok &= hipMemcpyAsync(drsel, rs.data(), rs.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
+        if (nseg) {
+            ok &= hipMemcpyAsync(base + o_sd, sdesc.data(), nseg * sizeof(SegDesc), hipMemcpyHostToDevice,
+                                 st) == hipSuccess;
+            ok &= hipMemcpyAsync(base + o_so, sord.data(), nseg * 4, hipMemcpyHostToDevice, st) == hipSuccess;
+            ok &= hipMemcpyAsync(base + o_sp, spairs.data(), spairs.size() * sizeof(SegPair),
+                                 hipMemcpyHostToDevice, st) == hipSuccess;
+        }
         if (!ok) {
             phip_set_error("phip_nw_banded_batch: upload failed");
             break;
@@ -850,6 +1350,20 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
             hipLaunchKernelGGL(fn, dim3(cc.npairs), dim3(64 * kRowsWaves), lds_r, st, cc);
             BOK(hipGetLastError());
         }
+        if (nseg) {
+            const size_t lds_s = (size_t)(((msz * msz + 3) & ~3) + (rows_rp ? msz : 0)) * 4;
+            auto fn = rows_rp ? (seg_rmax <= 16 ? k_banded_seg<16, true> : k_banded_seg<32, true>)
+                              : (seg_rmax <= 16 ? k_banded_seg<16, false> : k_banded_seg<32, false>);
+            const SegDesc *dsd = (const SegDesc *)(base + o_sd);
+            hipLaunchKernelGGL(fn, dim3((unsigned)((nseg + 3) / 4)), dim3(256), lds_s, st, c, dsd,
+                               (const int32_t *)(base + o_so), (int)nseg, (int2 *)(base + o_vec),
+                               (int *)(base + o_raw));
+            BOK(hipGetLastError());
+            hipLaunchKernelGGL(k_seg_join, dim3((unsigned)spairs.size()), dim3(64), 0, st, c,
+                               (const SegPair *)(base + o_sp), dsd, (const int2 *)(base + o_vec),
+                               (const int *)(base + o_raw));
+            BOK(hipGetLastError());
+        }
         BOK(hipEventRecord(phip_event(1), st));
         BOK(hipMemcpyAsync(out, c.out, (size_t)np * sizeof(phip_out_t), hipMemcpyDeviceToHost, st));
         BOK(hipStreamSynchronize(st));
@@ -858,5 +1372,59 @@ extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *ou
     } while (0);
     dev_free(base);
     phip_unlock();
-    return rc;
+    if (rc != 0 || spairs.empty()) return rc;
+    // pairs whose segments did not line up: again, without segments
+    std::vector<int64_t> fqo, fto;
+    std::vector<int32_t> fql, ftl, fidx;
+    for (const SegPair &p : spairs)
+        if (out[p.pair].status & PHIP_ST_ERR) {
+            fidx.push_back(p.pair);
+            fqo.push_back(b->qoff[p.pair]);
+            fto.push_back(b->toff[p.pair]);
+            fql.push_back(b->qlen[p.pair]);
+            ftl.push_back(b->tlen[p.pair]);
+        }
+    g_seg_stats[0] = (int)spairs.size();
+    g_seg_stats[1] = (int)nseg;
+    g_seg_stats[2] = (int)fidx.size();
+    if (fidx.empty()) return 0;
+    const double kms = phip_last_kernel_ms();
+    phip_batch_t sub = *b;
+    sub.npairs = (int)fidx.size();
+    sub.qoff = fqo.data();
+    sub.toff = fto.data();
+    sub.qlen = fql.data();
+    sub.tlen = ftl.data();
+    std::vector<phip_out_t> so(fidx.size());
+    rc = banded_impl(&sub, k, so.data(), false);
+    if (rc != 0) return rc;
+    for (size_t x = 0; x < fidx.size(); ++x) out[fidx[x]] = so[x];
+    phip_add_kernel_ms(kms);
+    return 0;
+}
+
+extern "C" int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *out)
+{
+    if (phip_init() != 0) return -1;
+    // s1 is required even with a PSSM, as in the reference (src/nw_banded.c:50)
+    if (b->npairs > 0 && (k <= 0 || b->open < 0 || b->gap < 0 || !b->q || !b->t)) {
+        phip_set_error("phip_nw_banded_batch: bad arguments");
+        return -1;
+    }
+    g_seg_stats[0] = g_seg_stats[1] = g_seg_stats[2] = 0;
+    return banded_impl(b, k, out, true);
+}
+
+extern "C" int phip_nw_banded_segment_cols(int cols)
+{
+    const int prev = g_seg_cols;
+    if (cols >= 0) g_seg_cols = cols;
+    return prev;
+}
+
+extern "C" void phip_nw_banded_last_stats(int *pairs, int *segments, int *redone)
+{
+    if (pairs) *pairs = g_seg_stats[0];
+    if (segments) *segments = g_seg_stats[1];
+    if (redone) *redone = g_seg_stats[2];
 }

@@ -45,6 +45,11 @@ void phip_note_kernel_time()
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, g_ev0, g_ev1) == hipSuccess) g_last_ms = g_last_main_ms = ms;
 }
+void phip_add_kernel_ms(double ms)
+{
+    g_last_ms += ms;
+    g_last_main_ms += ms;
+}
 void phip_lock() { pthread_mutex_lock(&g_lock); }
 void phip_unlock() { pthread_mutex_unlock(&g_lock); }
 hipStream_t phip_stream() { return g_stream; }

@@ -12,3 +12,5 @@ hipStream_t phip_stream();
 // 1: end), and recording their elapsed time as the last call's kernel time.
 hipEvent_t phip_event(int i);
 void phip_note_kernel_time();
+// adds ms to the last call's kernel times (a call made of two launches)
+void phip_add_kernel_ms(double ms);

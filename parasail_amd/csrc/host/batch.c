@@ -596,3 +596,13 @@ done:
     free(ql); free(tl); free(qo); free(to); free(res);
     return rc;
 }
+
+PHOST_EXPORT int parasail_hip_nw_banded_set_segment(int columns)
+{
+    return phip_nw_banded_segment_cols(columns);
+}
+
+PHOST_EXPORT void parasail_hip_nw_banded_last_stats(int *pairs, int *segments, int *redone)
+{
+    phip_nw_banded_last_stats(pairs, segments, redone);
+}

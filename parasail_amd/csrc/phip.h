@@ -146,6 +146,12 @@ int phip_nw_banded(const uint8_t *s1, int m, const uint8_t *s2, int n,
 
 /* banded global alignment of every pair of b (alg/kind ignored), band k */
 int phip_nw_banded_batch(const phip_batch_t *b, int k, phip_out_t *out);
+/* columns a segment of a long banded pair owns (0: no segments; < 0: keep);
+ * returns the previous value */
+int phip_nw_banded_segment_cols(int cols);
+/* the last banded batch: segmented pairs, segments, pairs aligned again
+ * without segments */
+void phip_nw_banded_last_stats(int *pairs, int *segments, int *redone);
 
 int phip_device_info(int *cus, int *clock_khz);
 

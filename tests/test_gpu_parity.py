@@ -536,12 +536,23 @@ def test_banded_batch(mats):
 
 
 def test_banded_rows(mats):
-    """Pairs of >= 512 query rows take the row-band kernel: bands from 3 to
-    1,800 columns (every rows-per-lane choice), the query longer and shorter
-    than the target, bands reaching past both ends of the query, targets of
-    1 to 40 columns, zero gap costs, scores in registers (DNA 2/-3, 5/2; a
-    PSSM of it) and in an LDS table (BLOSUM62 11/1; DNA with a gap extension
-    of 4 above an open of 2)."""
+    """Pairs of >= 512 query rows take the row-band kernel when segments are
+    off, and the segment kernel for bands of 384 to 2,048 slots when they
+    are on: bands from 3 to 1,800 columns (every rows-per-lane choice), the
+    query longer and shorter than the target, bands reaching past both ends
+    of the query, targets of 1 to 40 columns, zero gap costs, scores in
+    registers (DNA 2/-3, 5/2; a PSSM of it) and in an LDS table (BLOSUM62
+    11/1; DNA with a gap extension of 4 above an open of 2)."""
+    prev = P.nw_banded_set_segment(0)
+    try:
+        _banded_rows_cases(mats)
+        P.nw_banded_set_segment(1)
+        _banded_rows_cases(mats)
+    finally:
+        P.nw_banded_set_segment(prev)
+
+
+def _banded_rows_cases(mats):
     L = P.lib()
     prng = random.Random(21)
     shapes = [(512, 530, 5), (700, 640, 40), (3000, 3100, 20), (2500, 2400, 200),
@@ -584,6 +595,55 @@ def test_banded_rows(mats):
         assert r.contents.score == exp[0], (m, n, k)
         L.parasail_result_free(r)
         L.parasail_matrix_free(pp)
+
+
+def test_banded_segments(mats):
+    """Long banded pairs cut into segments that run at the same time and
+    are joined by the rank-convergence check: mutated DNA pairs (query longer
+    and shorter than the target, several segment lengths, segments off), a
+    protein pair over BLOSUM62 (LDS table), and unrelated pairs whose
+    segments do not join and are aligned again whole -- every score equal to
+    the oracle's banded DP."""
+    prng = random.Random(31)
+    pm, om = mats["dna"]
+    qs, ts = [], []
+    for i, m in enumerate((24000, 31000, 41000, 52000)):
+        s1 = rseq(prng, m, "ACGT")
+        s2 = mutate(prng, s1, "ACGT", sub=0.1, indel=0.02)
+        if i % 2:
+            s1, s2 = s2, s1
+        qs.append(s1)
+        ts.append(s2)
+    for k in (256, 200):
+        exp = [O.nw_banded(qs[i], ts[i], 5, 2, k, om)[:3] for i in range(len(qs))]
+        prev = P.nw_banded_set_segment(8192)
+        try:
+            for cols in (8192, 2048, 0):
+                P.nw_banded_set_segment(cols)
+                got = P.nw_banded_batch(qs, ts, 5, 2, k, pm)
+                pairs, segs, redone = P.nw_banded_last_stats()
+                if cols:
+                    assert pairs == len(qs) and segs > 2 * pairs and redone == 0, (cols, pairs, segs, redone)
+                else:
+                    assert pairs == 0
+                for i in range(len(qs)):
+                    assert (got["score"][i], got["end_query"][i], got["end_ref"][i]) == exp[i], \
+                        (k, cols, i, len(qs[i]), len(ts[i]))
+        finally:
+            P.nw_banded_set_segment(prev)
+    # unrelated pairs: the states never line up, the pairs are redone whole;
+    # a protein pair beside them joins
+    pb, ob = mats["blosum62"]
+    u = [(rseq(prng, 26000, "ACGT"), rseq(prng, 26100, "ACGT")) for _ in range(2)]
+    got = P.nw_banded_batch([a for a, _ in u], [b for _, b in u], 5, 2, 256, pm)
+    assert P.nw_banded_last_stats()[2] >= 1
+    for i, (a, b) in enumerate(u):
+        assert got["score"][i] == O.nw_banded(a, b, 5, 2, 256, om)[0], i
+    s1 = rseq(prng, 30000, AA)
+    s2 = mutate(prng, s1, AA, sub=0.15, indel=0.02)
+    got = P.nw_banded_batch([s1], [s2], 11, 1, 300, pb)
+    assert P.nw_banded_last_stats()[:2][0] == 1
+    assert got["score"][0] == O.nw_banded(s1, s2, 11, 1, 300, ob)[0]
 
 
 def test_long_nw_int32(mats):
