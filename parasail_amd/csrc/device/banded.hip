@@ -490,17 +490,26 @@ __device__ __forceinline__ void seg_chain(const BandCfg &c, const int *tab, cons
             botH -= open;
             // the row entering slot R-1 after step s' is colOff0 + s' - t +
             // tR + R; its letter and the step's target letter are loaded
-            // two steps ahead
-            int ta = ldt(s - t), tb = ldt(s + 1 - t);
-            int na = ldq(colOff0 + s - t + tR + R), nb = ldq(colOff0 + s + 1 - t + tR + R);
+            // two steps ahead, unclamped: in these steps the lanes' columns
+            // are at most n and the pad slots' rows at most m + 64 R, and
+            // both residue arrays end with zeroed slack
+            const uint8_t *const tcol = s2 - t;
+            const uint8_t *const qrow = s1 + (colOff0 - t + tR + R);
+            int ta = tcol[s], tb = tcol[s + 1];
+            int na = qrow[s], nb = qrow[s + 1];
+            // the DPP results: lane 0 (up) and lane 63 (left) keep -inf
+            int xH = BNEG, xF = BNEG, xHn = BNEG, xEn = BNEG;
             for (; s < e; s += R) {
+                const uint8_t *const tblk = tcol + s, *const qblk = qrow + s;
                 static_for<R>([&](auto uc) __attribute__((always_inline)) {
                     constexpr int U = decltype(uc)::value;
                     const int sp = s + U;
                     int &tq = (U & 1) ? tb : ta;
                     int &nq = (U & 1) ? nb : na;
                     const int tc4 = tq << 2;
-                    int Hup = shr1_or(BNEG, botH), F = shr1_or(BNEG, botF);
+                    xH = shr1_or(xH, botH);
+                    xF = shr1_or(xF, botF);
+                    int Hup = xH, F = xF;
                     auto slot = [&](int r, int HLn, int En) __attribute__((always_inline)) {
                         const int Wp = (int)__builtin_amdgcn_ubfe((unsigned)qp[(r + U) % R], tc4, 4);
                         const int Hdag = Hb[r] + Wp;
@@ -512,7 +521,8 @@ __device__ __forceinline__ void seg_chain(const BandCfg &c, const int *tab, cons
                     };
                     slot(0, Hb[1], Eb[1]);
                     qp[U % R] = prof[nq];
-                    const int xHn = shl1_or(BNEG, Hb[0]), xEn = shl1_or(BNEG, Eb[0]);
+                    xHn = shl1_or(xHn, Hb[0]);
+                    xEn = shl1_or(xEn, Eb[0]);
 #pragma unroll
                     for (int r = 1; r < R; ++r)
                         slot(r, r + 1 < R ? Hb[r + 1] : xHn, r + 1 < R ? Eb[r + 1] : xEn);
@@ -529,8 +539,8 @@ __device__ __forceinline__ void seg_chain(const BandCfg &c, const int *tab, cons
                             for (int r = 0; r < R; ++r) ven[r] = make_int2(Hb[r] + open, Eb[r]);
                         }
                     }
-                    tq = ldt(sp + 2 - t);
-                    nq = ldq(colOff0 + sp + 2 - t + tR + R);
+                    tq = tblk[U + 2];
+                    nq = qblk[U + 2];
                 });
             }
             // back to the slow step's state at step s
@@ -1016,6 +1026,9 @@ const int kBandRs[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
 // k_banded_seg: slots per lane from this set (even: the profile rotation
 // keeps a step's two prefetch registers on the same parity)
 const int kSegRs[] = {8, 10, 12, 14, 16, 20, 24, 32};
+// bytes after each residue array: the segment kernel's loads reach up to
+// 64 x 32 + 2 bytes past a pair's query and 2 past its target
+constexpr size_t kResidueSlack = 4096;
 // segment columns: 0 no segments, 1 sized to fill the device once, more:
 // about that many columns a segment
 int g_seg_cols = 1;
@@ -1247,7 +1260,8 @@ static int banded_impl(const phip_batch_t *b, int k, phip_out_t *out, bool allow
     const int64_t stride = maxcol + 2;
     size_t off = 0;
     auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
-    const size_t o_mat = slot(mbytes), o_q = slot((size_t)qtot), o_t = slot((size_t)ttot),
+    const size_t o_mat = slot(mbytes), o_q = slot((size_t)qtot + kResidueSlack),
+                 o_t = slot((size_t)ttot + kResidueSlack),
                  o_qo = slot((size_t)np * 8), o_to = slot((size_t)np * 8), o_ql = slot((size_t)np * 4),
                  o_tl = slot((size_t)np * 4), o_pairs = slot((size_t)np * 4), o_rsel = slot((size_t)np * 4),
                  o_out = slot((size_t)np * sizeof(phip_out_t)),
@@ -1292,6 +1306,10 @@ static int banded_impl(const phip_batch_t *b, int k, phip_out_t *out, bool allow
         ok &= hipMemcpyAsync((void *)c.mat, b->matrix.values, mbytes, hipMemcpyHostToDevice, st) == hipSuccess;
         if (qtot) ok &= hipMemcpyAsync((void *)c.q, b->q, (size_t)qtot, hipMemcpyHostToDevice, st) == hipSuccess;
         ok &= hipMemcpyAsync((void *)c.t, b->t, (size_t)ttot, hipMemcpyHostToDevice, st) == hipSuccess;
+        // the slack after each residue array (read by k_banded_seg's
+        // unclamped loads past a pair's end): code 0
+        ok &= hipMemsetAsync((void *)(c.q + qtot), 0, kResidueSlack, st) == hipSuccess;
+        ok &= hipMemsetAsync((void *)(c.t + ttot), 0, kResidueSlack, st) == hipSuccess;
         ok &= hipMemcpyAsync((void *)c.qoff, b->qoff, (size_t)np * 8, hipMemcpyHostToDevice, st) == hipSuccess;
         ok &= hipMemcpyAsync((void *)c.toff, b->toff, (size_t)np * 8, hipMemcpyHostToDevice, st) == hipSuccess;
         ok &= hipMemcpyAsync((void *)c.qlen, b->qlen, (size_t)np * 4, hipMemcpyHostToDevice, st) == hipSuccess;
