@@ -174,28 +174,10 @@ def cpu_baseline(batch, target_s=15.0):
                       "OpenMP over pairs, %.1f s" % (n2, secs)}
 
 
-def serving(nreq, qps):
-    """Config 4: a database of 1,000,000 proteins resident in HBM; nreq
-    Poisson requests at qps (10% PSSM), each SW of its query against the
-    whole database returning the top 50; latency from arrival to hits in
-    host memory.  Also one request of 300 residues searched alone."""
-    pm = P.blosum62()
-    t0 = time.perf_counter()
-    res, off, tl = WL.config4_db()
-    tgen = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    db = P.Database(res, off, tl, pm)
-    tup = time.perf_counter() - t0
-    del res
-    nres = int(tl.astype(np.int64).sum())
-    log("serving: database of %d targets, %.3e residues (generated %.1f s, resident in %.1f s)"
-        % (len(tl), nres, tgen, tup))
+def serve_load(db, pm, nres, nreq, qps):
+    """nreq Poisson requests at qps through one serving loop: latency of
+    each from its arrival to its hits in host memory."""
     reqs = WL.config4_queries(nreq=nreq, qps=qps)
-    q300 = WL.protein_bytes(np.random.default_rng(77), 300)
-    db.search(q300[:60], pm, 11, 1, 50)  # warm-up
-    t0 = time.perf_counter()
-    db.search(q300, pm, 11, 1, 50)
-    alone_s = time.perf_counter() - t0
     mats = []
     for _, q, noise in reqs:
         mats.append(P.blosum62_pssm(q, noise)[0] if noise is not None else pm)
@@ -215,25 +197,64 @@ def serving(nreq, qps):
         elif i < len(reqs):
             time.sleep(max(0.0, min(0.002, start + reqs[i][0] - P.now())))
     end = P.now()
+    _, deferred, max_active = srv.stats()
     srv.close()
-    db.close()
     for k, (_, _, noise) in enumerate(reqs):
         if noise is not None:
             P.lib().parasail_matrix_free(mats[k])
     cells = sum(len(q) for _, q, _ in reqs) * nres
     lt = np.array([lat[k] for k in range(len(reqs))])
-    return {"workload": "1,000,000-protein database (log-normal 350, 0.7, [30, 35000]) in HBM; "
-                        "Poisson queries (log-normal 300, 0.6, [20, 5000]), 10% PSSM; SW 11/1, "
-                        "top 50",
-            "requests": len(reqs), "qps": qps,
+    return {"qps": qps, "requests": len(reqs),
             "pssm_requests": sum(1 for r in reqs if r[2] is not None),
             "p50_ms": round(float(np.percentile(lt, 50)), 1),
             "p99_ms": round(float(np.percentile(lt, 99)), 1),
             "mean_ms": round(float(lt.mean()), 1),
             "gcups": round(cells / (end - start) / 1e9, 1),
-            "db_residues": nres,
-            "search_alone_300aa": {"ms": round(alone_s * 1e3, 1),
-                                   "gcups": round(300 * nres / alone_s / 1e9, 1)}}
+            "offered_gcups": round(cells / (reqs[-1][0] if reqs[-1][0] > 0 else 1.0) / 1e9, 1),
+            "max_in_flight": max_active, "admission_waits": deferred}
+
+
+def serving(nreq, qps, sweep):
+    """Config 4: a database of 1,000,000 proteins resident in HBM; nreq
+    Poisson requests at qps (10% PSSM), each SW of its query against the
+    whole database returning the top 50; latency from arrival to hits in
+    host memory.  The same at each rate of sweep (latency against load),
+    and one request of 300 residues searched alone."""
+    pm = P.blosum62()
+    t0 = time.perf_counter()
+    res, off, tl = WL.config4_db()
+    tgen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    db = P.Database(res, off, tl, pm)
+    tup = time.perf_counter() - t0
+    del res
+    nres = int(tl.astype(np.int64).sum())
+    log("serving: database of %d targets, %.3e residues (generated %.1f s, resident in %.1f s)"
+        % (len(tl), nres, tgen, tup))
+    q300 = WL.protein_bytes(np.random.default_rng(77), 300)
+    db.search(q300[:60], pm, 11, 1, 50)  # warm-up
+    t0 = time.perf_counter()
+    db.search(q300, pm, 11, 1, 50)
+    alone_s = time.perf_counter() - t0
+    main = serve_load(db, pm, nres, nreq, qps)
+    log("serving: %.0f QPS p50 %.1f ms p99 %.1f ms, %.0f GCUPS"
+        % (qps, main["p50_ms"], main["p99_ms"], main["gcups"]))
+    points = []
+    for x in sweep:
+        pt = main if x == qps else serve_load(db, pm, nres, nreq, x)
+        log("serving: %.0f QPS p50 %.1f ms p99 %.1f ms, %.0f GCUPS, %d in flight at most"
+            % (x, pt["p50_ms"], pt["p99_ms"], pt["gcups"], pt["max_in_flight"]))
+        points.append(pt)
+    db.close()
+    out = {"workload": "1,000,000-protein database (log-normal 350, 0.7, [30, 35000]) in HBM; "
+                       "Poisson queries (log-normal 300, 0.6, [20, 5000]), 10% PSSM; SW 11/1, "
+                       "top 50",
+           "db_residues": nres}
+    out.update(main)
+    out["search_alone_300aa"] = {"ms": round(alone_s * 1e3, 1),
+                                 "gcups": round(300 * nres / alone_s / 1e9, 1)}
+    out["qps_sweep"] = points
+    return out
 
 
 def stats_trace(peak32, npairs=100000):
@@ -421,6 +442,8 @@ def main():
     ap.add_argument("--no-stats", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--qps", type=float, default=8.0)
+    ap.add_argument("--qps-sweep", default="4,8,16,24",
+                    help="request rates of the serving sweep (comma-separated; empty: none)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once (HBM counter calibration)")
     a = ap.parse_args()
@@ -528,7 +551,8 @@ def main():
     if not a.no_long:
         line["long_pairs"] = long_pairs(cus.value * clk.value * 1e3 * 64)
     if not a.no_serve:
-        line["serving"] = serving(a.serve_requests, a.qps)
+        line["serving"] = serving(a.serve_requests, a.qps,
+                                  [float(x) for x in a.qps_sweep.split(",") if x])
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(batch)
     print(json.dumps(line), flush=True)

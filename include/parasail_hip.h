@@ -588,6 +588,12 @@ int parasail_hip_server_submit(parasail_hip_server_t *srv, int64_t id, const cha
 int parasail_hip_server_poll(parasail_hip_server_t *srv, int64_t *id, parasail_hip_hit_t *hits,
                              int *nhits, double *latency_ms, int wait);
 void parasail_hip_server_free(parasail_hip_server_t *srv);
+/* requests completed, requests that waited at admission because their
+ * device memory did not fit the budget beside those in flight (a request
+ * runs alone when nothing else is in flight), and the most requests in
+ * flight at once */
+int parasail_hip_server_stats(parasail_hip_server_t *srv, int64_t *completed, int64_t *deferred,
+                              int *max_active);
 /* monotonic clock (seconds) of the serving loop */
 double parasail_hip_now(void);
 

@@ -147,6 +147,12 @@ def lib():
                                                ctypes.POINTER(Hit), ctypes.POINTER(ctypes.c_int),
                                                ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.parasail_hip_server_free.argtypes = [ctypes.c_void_p]
+        L.parasail_hip_server_stats.restype = ctypes.c_int
+        L.parasail_hip_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                                ctypes.POINTER(ctypes.c_int64),
+                                                ctypes.POINTER(ctypes.c_int)]
+        L.parasail_hip_nw_banded_set_segment.restype = ctypes.c_int
+        L.parasail_hip_nw_banded_set_segment.argtypes = [ctypes.c_int]
         L.parasail_hip_now.restype = ctypes.c_double
         L.parasail_hip_last_main_kernel_ms.restype = ctypes.c_double
         L.parasail_hip_last_error.restype = ctypes.c_char_p
@@ -379,6 +385,12 @@ class Server:
         if rc < 0:
             raise RuntimeError("request %d failed" % rid.value)
         return rid.value, [hits[i].as_tuple() for i in range(nh.value)], lat.value
+
+    def stats(self):
+        """(completed, deferred at admission, most in flight at once)."""
+        c, d, a = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        lib().parasail_hip_server_stats(self.h, ctypes.byref(c), ctypes.byref(d), ctypes.byref(a))
+        return c.value, d.value, a.value
 
     def close(self):
         if self.h:

@@ -519,6 +519,7 @@ struct Req {
     hipEvent_t done{};
     phip_hit_t *hhits = nullptr;  // pinned
     int slot = -1;
+    bool deferred = false;  // waited for memory at admission
 };
 
 int choose_R(int m, int *nb16, int *nb64)
@@ -629,7 +630,9 @@ int req_prepare(Req *r, const phip_db_t *db, const phip_query_t *q, int k)
     return 0;
 }
 
-int req_alloc(Req *r, const phip_db_t *db)
+// Lays out the request's device memory (profile, per-target results, band
+// rows, top-k sort scratch...) into r->bytes and, if allocate, allocates it.
+int req_alloc(Req *r, const phip_db_t *db, bool allocate)
 {
     const int n = db->n;
     const int nch = (int)r->chunk_w0.size();
@@ -651,6 +654,7 @@ int req_alloc(Req *r, const phip_db_t *db)
                  o_scr = slot(per_lane * 64), o_keys = slot((size_t)n * 8),
                  o_sorted = slot((size_t)n * 8), o_temp = slot(temp), o_hits = slot((size_t)r->k * sizeof(phip_hit_t));
     r->bytes = o;
+    if (!allocate) return 0;
     r->base = (char *)dev_alloc(o);
     if (!r->base) return -1;
     char *b = r->base;
@@ -786,7 +790,7 @@ extern "C" int phip_db_search(phip_db_t *db, const phip_query_t *q, int k, phip_
     int rc = -1;
     hipStream_t st = phip_stream();
     do {
-        if (req_alloc(&r, db) != 0) break;
+        if (req_alloc(&r, db, true) != 0) break;
         if (req_start(&r, db, st) != 0) break;
         bool ok = true;
         for (int ch = 0; ch < (int)r.chunk_w0.size() && ok; ++ch)
@@ -833,6 +837,8 @@ struct phip_server {
     pthread_cond_t cv = PTHREAD_COND_INITIALIZER;
     bool stop = false, started = false;
     int64_t completed = 0;
+    int64_t deferred = 0;  // requests that waited for memory at admission
+    int max_active = 0;    // most requests on the device at once
     double busy_cells = 0;
 };
 
@@ -862,19 +868,20 @@ void *server_main(void *arg)
                 if (!s->active[i]) { free_slot = i; break; }
             if (free_slot < 0) break;
             Pending p = s->pending.front();
-            pthread_mutex_unlock(&s->mu);
-            phip_lock();
-            int rc = req_alloc(p.r, s->db);
+            // its footprint against the budget first: with others running,
+            // a request that does not fit waits (oldest first) for one of
+            // them to finish; alone, it always runs
+            int rc = p.r->bytes || req_alloc(p.r, s->db, false) == 0 ? 0 : -1;
             if (rc == 0 && s->in_use + p.r->bytes > s->budget && count_active(s) > 0) {
-                // not enough memory while others run: wait for one to finish
-                dev_free(p.r->base);
-                p.r->base = nullptr;
-                if (p.r->hhits) (void)hipHostFree(p.r->hhits);
-                p.r->hhits = nullptr;
-                phip_unlock();
-                pthread_mutex_lock(&s->mu);
+                if (!p.r->deferred) {
+                    p.r->deferred = true;
+                    s->deferred++;
+                }
                 break;
             }
+            pthread_mutex_unlock(&s->mu);
+            phip_lock();
+            if (rc == 0) rc = req_alloc(p.r, s->db, true);
             if (rc == 0) rc = req_start(p.r, s->db, s->streams[free_slot]);
             phip_unlock();
             pthread_mutex_lock(&s->mu);
@@ -897,13 +904,14 @@ void *server_main(void *arg)
             p.r->slot = free_slot;
             s->in_use += p.r->bytes;
             s->active[free_slot] = p.r;
+            s->max_active = std::max(s->max_active, count_active(s));
             progressed = true;
         }
         pthread_mutex_unlock(&s->mu);
-        // keep every active request's stream fed with chunks; the grid is
-        // shared among the active requests
-        const int na = std::max(1, count_active(s));
-        const int blocks = std::max(8, search_blocks() / na);
+        // keep every active request's stream fed with chunks, each launched
+        // over the whole grid: the device runs the streams' kernels side by
+        // side as CUs free up
+        const int blocks = search_blocks();
         for (int i = 0; i < s->slots; ++i) {
             Req *r = s->active[i];
             if (!r) continue;
@@ -1024,6 +1032,17 @@ extern "C" int phip_server_poll(phip_server_t *s, int64_t *id, phip_hit_t *hits,
 }
 
 extern "C" double phip_now(void) { return now_s(); }
+
+extern "C" int phip_server_stats(phip_server_t *s, int64_t *completed, int64_t *deferred, int *max_active)
+{
+    if (!s) return -1;
+    pthread_mutex_lock(&s->mu);
+    if (completed) *completed = s->completed;
+    if (deferred) *deferred = s->deferred;
+    if (max_active) *max_active = s->max_active;
+    pthread_mutex_unlock(&s->mu);
+    return 0;
+}
 
 extern "C" void phip_server_free(phip_server_t *s)
 {

@@ -201,4 +201,11 @@ PHOST_EXPORT void parasail_hip_server_free(parasail_hip_server_t *s)
     free(s);
 }
 
+PHOST_EXPORT int parasail_hip_server_stats(parasail_hip_server_t *s, int64_t *completed,
+                                           int64_t *deferred, int *max_active)
+{
+    if (!s) return -1;
+    return phip_server_stats(s->srv, completed, deferred, max_active);
+}
+
 PHOST_EXPORT double parasail_hip_now(void) { return phip_now(); }

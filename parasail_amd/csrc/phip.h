@@ -193,6 +193,9 @@ int phip_server_submit(phip_server_t *s, int64_t id, const phip_query_t *q, int 
 int phip_server_poll(phip_server_t *s, int64_t *id, phip_hit_t *hits, int *nhits,
                      double *latency_ms, int wait);
 void phip_server_free(phip_server_t *s);
+/* requests completed, requests that waited for memory at admission, and
+ * the most requests on the device at once */
+int phip_server_stats(phip_server_t *s, int64_t *completed, int64_t *deferred, int *max_active);
 double phip_now(void);
 
 /* timing of the last phip_align call's dominant kernel (ms, HIP events) */

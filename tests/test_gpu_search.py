@@ -135,10 +135,47 @@ def test_server_results_alone_equal_in_mix(db_case):
             rid, hits, lat = srv.poll(50)
             assert lat > 0
             got[rid] = hits
+        completed, deferred, max_active = srv.stats()
     finally:
         srv.close()
     for i in range(len(reqs)):
         assert got[i] == alone[i], i
+    # the default budget holds them all: they ran side by side
+    assert completed == len(reqs) and deferred == 0 and max_active > 1, (completed, deferred, max_active)
     for q, mx in reqs:
         if q is None:
             P.lib().parasail_matrix_free(mx)
+
+
+def test_server_admission_waits_for_memory(db_case):
+    """A memory budget smaller than two requests' footprints: each request
+    waits at admission until the one in flight finishes (one at a time), and
+    every request still returns what it returns searched alone."""
+    rng = np.random.default_rng(34)
+    db, pm = db_case["db"], db_case["pm"]
+    # the first request is long, so the others arrive while it runs
+    qs = [WL.protein_bytes(rng, m) for m in (4000, 60, 900, 130, 400)]
+    alone = [db.search(q, pm, 11, 1, 20) for q in qs]
+    for budget in (1, 0):
+        srv = P.Server(db, mem_budget=budget)
+        try:
+            t0 = P.now()
+            for i, q in enumerate(qs):
+                srv.submit(i, q, pm, 11, 1, 20, t0)
+            got, order = {}, []
+            for _ in qs:
+                rid, hits, _ = srv.poll(20)
+                got[rid] = hits
+                order.append(rid)
+            completed, deferred, max_active = srv.stats()
+        finally:
+            srv.close()
+        for i in range(len(qs)):
+            assert got[i] == alone[i], (budget, i)
+        assert completed == len(qs)
+        if budget == 1:
+            # one request on the device at a time, admitted oldest first
+            assert max_active == 1 and deferred >= 1, (max_active, deferred)
+            assert order == list(range(len(qs))), order
+        else:
+            assert deferred == 0 and 1 < max_active <= 4, (max_active, deferred)
