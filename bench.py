@@ -56,6 +56,11 @@ def make_resident(batch, matrix):
     L.parasail_hip_last_pass_ms.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double)]
     L.parasail_hip_batch_free.argtypes = [ctypes.c_void_p]
+    L.parasail_hip_batch_set_graph.restype = ctypes.c_int
+    L.parasail_hip_batch_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.parasail_hip_batch_graph_stats.restype = ctypes.c_int
+    L.parasail_hip_batch_graph_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int)]
     I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
     h = L.parasail_hip_batch_create(batch.q, batch.qoff.ctypes.data_as(I64),
                                     batch.qlen.ctypes.data_as(I32), batch.t,
@@ -494,6 +499,18 @@ def main():
             mms[alg].append(mm)
         log("step %d done" % s)
     elapsed = time.perf_counter() - t0
+    # the multi-pass call is one HIP graph launch after its capture; the
+    # same steps as separate launches, for what the graph saves
+    L = P.lib()
+    gcap, grep_ = ctypes.c_int(), ctypes.c_int()
+    L.parasail_hip_batch_graph_stats(h, ctypes.byref(gcap), ctypes.byref(grep_))
+    L.parasail_hip_batch_set_graph(h, 0)
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        mp.run()
+    elapsed_ng = time.perf_counter() - t0
+    log("multi-pass step: %.2f ms as a graph replay, %.2f ms as launches"
+        % (elapsed / a.steps * 1e3, elapsed_ng / a.steps * 1e3))
     P.lib().parasail_hip_batch_free(h)
     # the same batch from host arrays in one parasail_hip_align_batch call
     # per algorithm: encoding, upload, planning, kernels and download timed
@@ -524,6 +541,8 @@ def main():
                                "BLOSUM62 11/1, SW+NW+SG score-only, int16-first sat",
                    "pairs": len(batch), "cells_per_alg": cells},
         "gcups_kernel_by_alg": per_alg,
+        "graph": {"captures": gcap.value, "replays": grep_.value,
+                  "ms_per_step_without_graph": round(elapsed_ng / a.steps * 1e3, 2)},
         "cold_batch": dict(cold, includes="residues from host arrays: encoding, upload, "
                                           "planning, kernels, download; first call of "
                                           "each algorithm"),

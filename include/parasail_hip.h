@@ -545,6 +545,15 @@ int parasail_hip_batch_align_multi(parasail_hip_batch_t *batch, int nalg,
                                    parasail_hip_batch_result_t *out);
 int parasail_hip_last_pass_ms(int k, double *kernel_ms, double *main_kernel_ms);
 void parasail_hip_batch_free(parasail_hip_batch_t *batch);
+/* A parasail_hip_batch_align_multi call that repeats the previous call's
+ * algorithms and gaps on the wavefront tier (whose plans that call built)
+ * is captured into a HIP graph -- every pass's kernels, result packing and
+ * download -- and later such calls replay it with one launch.  on = 0
+ * turns this off for the batch (returns the previous setting);
+ * graph_stats gives the graphs captured and the calls that replayed one.
+ * Neither has a counterpart in the reference. */
+int parasail_hip_batch_set_graph(parasail_hip_batch_t *batch, int on);
+int parasail_hip_batch_graph_stats(parasail_hip_batch_t *batch, int *captures, int *replays);
 
 /* ---- database search and serving (no reference counterpart; the
  * reference's callers run parasail_sw_striped_profile_* over every target

@@ -171,11 +171,23 @@ struct phip_resident {
     hipEvent_t pev[kPasses][5] = {};  // start, end, main start, main end, downloaded
     int npass = 0;
     bool pass_wave[kPasses] = {};
+    // the multi-pass call as a HIP graph: a call repeating the previous
+    // call's passes (whose plans that call built) is captured once, then
+    // replayed by every later such call
+    bool graph_on = true;
+    int gkey[3 + 2 * kPasses] = {};
+    bool gkey_set = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool gwave[kPasses] = {};
+    int graph_captures = 0, graph_replays = 0;
 };
 
 typedef struct phip_resident phip_resident_t;
 
 static void resident_free(phip_resident_t *r);
+static int resident_enqueue4(phip_resident_t *r, int npass, const int *alg, const int *sg_flags,
+                             int open, int gap);
 
 static phip_resident_t *resident_create(const phip_batch_t *b)
 {
@@ -278,6 +290,8 @@ static int resident_host_buffers(phip_resident_t *r)
 static void resident_free(phip_resident_t *r)
 {
     if (!r) return;
+    if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
     for (int k = 0; k < phip_resident_t::kPasses; ++k) {
         if (r->pd4[k]) dev_free(r->pd4[k]);
         if (r->ph4[k]) (void)hipHostFree(r->ph4[k]);
@@ -1083,22 +1097,64 @@ extern "C" int phip_resident_launch4(void *handle, int npass, const int *alg, co
         if (rc || !r->pd4[k] || !r->ph4[k]) {
             phip_set_error("phip_resident_launch4: out of memory");
             rc = -1;
-            break;
         }
-        hipEvent_t *ev = r->pev[k];
-        if (hipEventRecord(ev[0], g_stream) != hipSuccess) { rc = -1; break; }
-        // the main-kernel events stay unrecorded unless the wavefront runs
-        rc = resident_run(r, alg[k], PHIP_SCORE, sg_flags[k], open, gap, 0, ev[2], ev[3]);
-        if (rc) break;
-        r->pass_wave[k] = r->last_wave;
-        hipLaunchKernelGGL(k_pack_out4, dim3((r->np + 255) / 256), dim3(256), 0, g_stream, r->dout,
-                           r->pd4[k], r->np);
-        if (hipGetLastError() != hipSuccess || hipEventRecord(ev[1], g_stream) != hipSuccess ||
-            hipMemcpyAsync(r->ph4[k], r->pd4[k], (size_t)r->np * sizeof(phip_out4_t),
-                           hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
-            hipEventRecord(ev[4], g_stream) != hipSuccess) {
-            phip_set_error("phip_resident_launch4: launch failed");
-            rc = -1;
+    }
+    if (rc) {
+        pthread_mutex_unlock(&g_lock);
+        return -1;
+    }
+    int key[3 + 2 * phip_resident_t::kPasses] = {npass, open, gap};
+    for (int k = 0; k < npass; ++k) {
+        key[3 + 2 * k] = alg[k];
+        key[4 + 2 * k] = sg_flags[k];
+    }
+    const bool same = r->gkey_set && memcmp(key, r->gkey, sizeof(key)) == 0;
+    if (!same && r->gexec) {
+        (void)hipGraphExecDestroy(r->gexec);
+        (void)hipGraphDestroy(r->graph);
+        r->gexec = nullptr;
+        r->graph = nullptr;
+    }
+    memcpy(r->gkey, key, sizeof(key));
+    r->gkey_set = true;
+    if (r->graph_on && same && r->gexec) {
+        // the same passes as the captured call: one graph launch
+        if (hipGraphLaunch(r->gexec, g_stream) != hipSuccess) {
+            phip_set_error("phip_resident_launch4: graph launch failed");
+            pthread_mutex_unlock(&g_lock);
+            return -1;
+        }
+        memcpy(r->pass_wave, r->gwave, sizeof(r->gwave));
+        r->graph_replays++;
+        r->npass = npass;
+        return 0;
+    }
+    // the same passes as the previous call, all on the wavefront tier (their
+    // plans exist and own every buffer the launches touch, so the passes are
+    // launches and copies only): capture them into a graph
+    bool all_wave = true;
+    for (int k = 0; k < npass; ++k) all_wave = all_wave && r->pass_wave[k];
+    const bool capture = r->graph_on && same && all_wave &&
+                         hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    rc = resident_enqueue4(r, npass, alg, sg_flags, open, gap);
+    if (capture) {
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(g_stream, &g);
+        hipGraphExec_t ge = nullptr;
+        if (rc == 0 && ec == hipSuccess && g &&
+            hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess &&
+            hipGraphLaunch(ge, g_stream) == hipSuccess) {
+            r->graph = g;
+            r->gexec = ge;
+            memcpy(r->gwave, r->pass_wave, sizeof(r->gwave));
+            r->graph_captures++;
+        } else {
+            // a step the graph cannot hold: run the passes as launches
+            if (ge) (void)hipGraphExecDestroy(ge);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            r->graph_on = false;
+            rc = resident_enqueue4(r, npass, alg, sg_flags, open, gap);
         }
     }
     if (rc) {
@@ -1107,6 +1163,55 @@ extern "C" int phip_resident_launch4(void *handle, int npass, const int *alg, co
         return -1;
     }
     r->npass = npass;
+    return 0;
+}
+
+// The passes of phip_resident_launch4 on g_stream: each pass's kernels, its
+// results packed and downloaded, and its events.  Launches and copies only.
+static int resident_enqueue4(phip_resident_t *r, int npass, const int *alg, const int *sg_flags,
+                             int open, int gap)
+{
+    int rc = 0;
+    for (int k = 0; k < npass && rc == 0; ++k) {
+        hipEvent_t *ev = r->pev[k];
+        // the pass's events are recorded as nodes of a captured graph too
+        // (external records), so every replay records them again
+        if (hipEventRecordWithFlags(ev[0], g_stream, hipEventRecordExternal) != hipSuccess) { rc = -1; break; }
+        // the main-kernel events stay unrecorded unless the wavefront runs
+        rc = resident_run(r, alg[k], PHIP_SCORE, sg_flags[k], open, gap, 0, ev[2], ev[3]);
+        if (rc) break;
+        r->pass_wave[k] = r->last_wave;
+        hipLaunchKernelGGL(k_pack_out4, dim3((r->np + 255) / 256), dim3(256), 0, g_stream, r->dout,
+                           r->pd4[k], r->np);
+        if (hipGetLastError() != hipSuccess ||
+            hipEventRecordWithFlags(ev[1], g_stream, hipEventRecordExternal) != hipSuccess ||
+            hipMemcpyAsync(r->ph4[k], r->pd4[k], (size_t)r->np * sizeof(phip_out4_t),
+                           hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+            hipEventRecordWithFlags(ev[4], g_stream, hipEventRecordExternal) != hipSuccess) {
+            phip_set_error("phip_resident_launch4: launch failed");
+            rc = -1;
+        }
+    }
+    return rc;
+}
+
+extern "C" int phip_resident_set_graph(void *handle, int on)
+{
+    phip_resident_t *r = (phip_resident_t *)handle;
+    if (!r) return -1;
+    pthread_mutex_lock(&g_lock);
+    const int prev = r->graph_on ? 1 : 0;
+    r->graph_on = on != 0;
+    pthread_mutex_unlock(&g_lock);
+    return prev;
+}
+
+extern "C" int phip_resident_graph_stats(void *handle, int *captures, int *replays)
+{
+    phip_resident_t *r = (phip_resident_t *)handle;
+    if (!r) return -1;
+    if (captures) *captures = r->graph_captures;
+    if (replays) *replays = r->graph_replays;
     return 0;
 }
 

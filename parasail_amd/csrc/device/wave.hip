@@ -982,12 +982,12 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream, hipEvent_
         cfg.sg_flags = P->sg_flags;
         KFn fn = P->sw ? k_wave<true> : k_wave<false>;
         HIP_OK(hipEventRecord(g_kw0, stream));
-        if (e0) HIP_OK(hipEventRecord(e0, stream));
+        if (e0) HIP_OK(hipEventRecordWithFlags(e0, stream, hipEventRecordExternal));
         const size_t lds = (size_t)((cfg.tabn + 3) & ~3) * 4 + (P->sw ? 2 * kSnapRows * 256 * 2 : 0);
         hipLaunchKernelGGL(fn, dim3(P->nblocks), dim3(256), lds, stream, cfg);
         HIP_OK(hipGetLastError());
         HIP_OK(hipEventRecord(g_kw1, stream));
-        if (e1) HIP_OK(hipEventRecord(e1, stream));
+        if (e1) HIP_OK(hipEventRecordWithFlags(e1, stream, hipEventRecordExternal));
         g_kw_recorded = true;
         if (P->sw) {
             hipLaunchKernelGGL(k_sw_reduce, dim3((P->nfast + 15) / 16), dim3(256), 0, stream,

@@ -489,6 +489,16 @@ PHOST_EXPORT void parasail_hip_batch_free(parasail_hip_batch_t *hb)
     free(hb);
 }
 
+PHOST_EXPORT int parasail_hip_batch_set_graph(parasail_hip_batch_t *hb, int on)
+{
+    return hb ? phip_resident_set_graph(hb->dev, on) : -1;
+}
+
+PHOST_EXPORT int parasail_hip_batch_graph_stats(parasail_hip_batch_t *hb, int *captures, int *replays)
+{
+    return hb ? phip_resident_graph_stats(hb->dev, captures, replays) : -1;
+}
+
 /* ---- banded global alignment of many pairs ---- */
 
 PHOST_EXPORT int parasail_hip_nw_banded_batch(const char *s1, const int64_t *s1_off,

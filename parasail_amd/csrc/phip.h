@@ -134,6 +134,11 @@ int phip_resident_launch4(void *handle, int npass, const int *alg, const int *sg
 int phip_resident_wait4(void *handle, int k, const phip_out4_t **res, double *kernel_ms,
                         double *main_ms);
 void phip_resident_end4(void *handle);
+/* whether a multi-pass call repeating the previous one is captured into a
+ * HIP graph and replayed (default on); returns the previous setting */
+int phip_resident_set_graph(void *handle, int on);
+/* graphs captured, and multi-pass calls that replayed one */
+int phip_resident_graph_stats(void *handle, int *captures, int *replays);
 /* out == NULL: into the handle's pinned buffer, phip_resident_results() */
 int phip_resident_align(void *handle, int alg, int sg_flags, int open, int gap, phip_out_t *out);
 const phip_out_t *phip_resident_results(void *handle);

@@ -809,6 +809,76 @@ def test_resident_batch_single_and_multi(mats):
         L.parasail_hip_batch_free(h)
 
 
+def test_resident_batch_graph_replays(mats):
+    """A multi-pass call repeating the previous one is captured into a HIP
+    graph and later ones replay it: every call's results equal the eager
+    single-algorithm calls', with eager work on the same batch and device
+    (single passes, a per-call alignment) between the replays."""
+    from parasail_amd import workloads as WL
+    pm, om = mats["blosum62"]
+    b = WL.config1(seed=22, npairs=3000)
+    n = len(b)
+    L = _resident_api()
+    L.parasail_hip_batch_graph_stats.restype = ctypes.c_int
+    L.parasail_hip_batch_graph_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int)]
+    L.parasail_hip_batch_set_graph.restype = ctypes.c_int
+    L.parasail_hip_batch_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    I64, I32 = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)
+    qoff = np.ascontiguousarray(b.qoff, dtype=np.int64)
+    toff = np.ascontiguousarray(b.toff, dtype=np.int64)
+    qlen = np.ascontiguousarray(b.qlen, dtype=np.int32)
+    tlen = np.ascontiguousarray(b.tlen, dtype=np.int32)
+    h = L.parasail_hip_batch_create(b.q, qoff.ctypes.data_as(I64), qlen.ctypes.data_as(I32), b.t,
+                                    toff.ctypes.data_as(I64), tlen.ctypes.data_as(I32), n, pm)
+    assert h
+    order = ("sw", "nw", "sg")
+    try:
+        single = {}
+        for name in order:
+            arrs, br = _bufs(n)
+            assert L.parasail_hip_batch_align(h, name.encode(), 11, 1, ctypes.byref(br)) == 0
+            single[name] = arrs
+        names = (ctypes.c_char_p * len(order))(*[x.encode() for x in order])
+        prev_times = None
+        for call in range(6):
+            outs = (P.BatchResult * len(order))()
+            arrs = []
+            for k in range(len(order)):
+                a, br = _bufs(n)
+                arrs.append(a)
+                outs[k] = br
+            assert L.parasail_hip_batch_align_multi(h, len(order), names, 11, 1, outs) == 0
+            for k, name in enumerate(order):
+                for key in ("score", "end_query", "end_ref", "flag"):
+                    assert (arrs[k][key] == single[name][key]).all(), (call, name, key)
+            # each call's pass events are recorded anew (a replay whose events
+            # kept an earlier call's state would let the host read results
+            # before their download ended)
+            times = []
+            for k in range(len(order)):
+                km, mm = ctypes.c_double(), ctypes.c_double()
+                assert L.parasail_hip_last_pass_ms(k, ctypes.byref(km), ctypes.byref(mm)) == 0
+                assert km.value > 0 and 0 < mm.value <= km.value + 1e-3, (call, k, km.value, mm.value)
+                times.append((km.value, mm.value))
+            assert times != prev_times, (call, times)
+            prev_times = times
+            # eager work between replays: a single pass over the same batch
+            # and a per-call alignment
+            a, br = _bufs(n)
+            assert L.parasail_hip_batch_align(h, b"nw", 11, 1, ctypes.byref(br)) == 0
+            assert (a["score"] == single["nw"]["score"]).all(), call
+            r = P.align("parasail_sw_striped_hip_64_16", "HEAGAWGHEE", "PAWHEAE", 10, 1, pm)
+            assert r["score"] == O.align(O.SW, O.SCORE, "HEAGAWGHEE", "PAWHEAE", 10, 1, om)[0][0]
+        cap, rep = ctypes.c_int(), ctypes.c_int()
+        assert L.parasail_hip_batch_graph_stats(h, ctypes.byref(cap), ctypes.byref(rep)) == 0
+        # call 0 repeats no call, call 1 is captured, calls 2-5 replay
+        assert (cap.value, rep.value) == (1, 4), (cap.value, rep.value)
+        assert L.parasail_hip_batch_set_graph(h, 0) == 1
+    finally:
+        L.parasail_hip_batch_free(h)
+
+
 # ------------------------------------------------ saturation windows
 #
 # The reference's fixed-width kernels report saturation by predicates of
