@@ -445,6 +445,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-long", action="store_true")
     ap.add_argument("--no-stats", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="plan config 1 with the instruction-count step costs")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--qps", type=float, default=8.0)
     ap.add_argument("--qps-sweep", default="4,8,16,24",
@@ -481,9 +483,41 @@ def main():
     log("workload: %d pairs, %.3e cells per algorithm (generated in %.1f s)"
         % (len(batch), cells, time.perf_counter() - t0))
     m = P.blosum62()
-    h = make_resident(batch, m)
     bufs = {alg: {k: np.zeros(len(batch), dtype=np.int32)
                   for k in ("score", "end_query", "end_ref", "flag")} for alg in algs}
+    # the planner's step costs measured on this device (parasail_hip_autotune)
+    # against the instruction count it uses untuned: a few steps of each plan
+    tune = None
+    if not a.no_autotune:
+        L = P.lib()
+        L.parasail_hip_autotune.restype = ctypes.c_int
+        L.parasail_hip_autotune.argtypes = [P.MP, ctypes.c_int, ctypes.c_int]
+        L.parasail_hip_autotune_table.restype = ctypes.c_int
+        L.parasail_hip_autotune_table.argtypes = [ctypes.POINTER(ctypes.c_int),
+                                                  ctypes.POINTER(ctypes.c_double),
+                                                  ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        h0 = make_resident(batch, m)
+        mp0 = MultiPass(h0, algs, bufs)
+        for _ in range(2):
+            mp0.run()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            mp0.run()
+        untuned = (time.perf_counter() - t0) / 3 * 1e3
+        L.parasail_hip_batch_free(h0)
+        t0 = time.perf_counter()
+        if L.parasail_hip_autotune(m, 11, 1) != 0:
+            raise RuntimeError("parasail_hip_autotune failed")
+        tune_s = time.perf_counter() - t0
+        rows = (ctypes.c_int * 16)()
+        swc, nwc = (ctypes.c_double * 16)(), (ctypes.c_double * 16)()
+        k = L.parasail_hip_autotune_table(rows, swc, nwc, 16)
+        tune = {"seconds": round(tune_s, 2), "untuned_ms_per_step": round(untuned, 2),
+                "rows_per_lane": [rows[i] for i in range(k)],
+                "sw_ns_per_wave_step": [round(swc[i], 4) for i in range(k)],
+                "nw_ns_per_wave_step": [round(nwc[i], 4) for i in range(k)]}
+        log("autotune: %.2f s; untuned step %.2f ms" % (tune_s, untuned))
+    h = make_resident(batch, m)
     # a step: SW, NW and SG over the resident batch in one library call, each
     # pass's download and conversion overlapping the next pass's kernels
     mp = MultiPass(h, algs, bufs)
@@ -543,6 +577,7 @@ def main():
         "gcups_kernel_by_alg": per_alg,
         "graph": {"captures": gcap.value, "replays": grep_.value,
                   "ms_per_step_without_graph": round(elapsed_ng / a.steps * 1e3, 2)},
+        "autotune": dict(tune, tuned_ms_per_step=round(ms_per_step, 2)) if tune else None,
         "cold_batch": dict(cold, includes="residues from host arrays: encoding, upload, "
                                           "planning, kernels, download; first call of "
                                           "each algorithm"),

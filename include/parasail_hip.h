@@ -554,6 +554,18 @@ void parasail_hip_batch_free(parasail_hip_batch_t *batch);
  * Neither has a counterpart in the reference. */
 int parasail_hip_batch_set_graph(parasail_hip_batch_t *batch, int on);
 int parasail_hip_batch_graph_stats(parasail_hip_batch_t *batch, int *captures, int *replays);
+/* Measured tuning of the score-only batch planner (no counterpart in the
+ * reference).  The planner picks, per pair, the rows per lane R of the
+ * packed-int16 wavefront kernel by the cost of a step at each R; untuned
+ * that cost comes from an instruction count.  parasail_hip_autotune times
+ * the kernel on the device at every R (SW, and NW for NW/SG) over synthetic
+ * pairs scored with matrix, open and gap, and installs the measured costs
+ * for every later plan; _table reads them back (ns per wave-step; returns
+ * how many, 0 if untuned); _reset returns to the instruction count.  Plans
+ * already built keep their shapes.  0, or -1 with a message. */
+int parasail_hip_autotune(const parasail_matrix_t *matrix, int open, int gap);
+int parasail_hip_autotune_table(int *rows, double *sw_cost, double *nw_cost, int cap);
+void parasail_hip_autotune_reset(void);
 
 /* ---- database search and serving (no reference counterpart; the
  * reference's callers run parasail_sw_striped_profile_* over every target

@@ -581,26 +581,38 @@ constexpr int kRmax = 20;
 
 struct Shape {
     int R, nb;
-    double cost;  // SIMD cycles of the job's wave issuing alone
+    double cost;  // time of the job's wave (cost of a step times steps)
 };
 
-// VALU instructions per lane: ~13 per row and ~30 per step; 4 cycles each
-double job_cycles(int T, int R, int nb, int n)
+// Cost of one step of a wave with R rows per lane: measured on the device
+// (phip_wave_autotune: ns per wave-step, [1] SW, [0] NW/SG) or, untuned,
+// the instruction model -- ~13 VALU instructions per row and ~30 per step,
+// 4 cycles each.  Only ratios matter to the planner.
+double g_step_cost[2][kRmax + 1];
+bool g_tuned = false;
+
+double step_cost(bool sw, int R)
 {
-    return (double)nb * (n + T - 1) * (13.0 * R + 30.0) * 4.0;
+    if (g_tuned && g_step_cost[sw][R] > 0) return g_step_cost[sw][R];
+    return (13.0 * R + 30.0) * 4.0;
+}
+
+double job_cycles(bool sw, int T, int R, int nb, int n)
+{
+    return (double)nb * (n + T - 1) * step_cost(sw, R);
 }
 
 // Band height T*R and band count for a query of m rows against n columns:
 // the cheapest (rows computed x steps) cover of the query.
-Shape choose(int T, int m, int n)
+Shape choose(bool sw, int T, int m, int n)
 {
     Shape best{kRmax, (m + T * kRmax - 1) / (T * kRmax), 0};
-    best.cost = job_cycles(T, best.R, best.nb, n);
+    best.cost = job_cycles(sw, T, best.R, best.nb, n);
     const int maxnb = std::max(1, (m + T * 4 - 1) / (T * 4));
     for (int nb = 1; nb <= maxnb && nb <= 512; ++nb) {
         for (int R : kRs) {
             if ((int64_t)nb * T * R < m) continue;
-            const double cost = job_cycles(T, R, nb, n);
+            const double cost = job_cycles(sw, T, R, nb, n);
             if (cost < best.cost) best = Shape{R, nb, cost};
             break;
         }
@@ -699,7 +711,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
         P->maxn = std::max(P->maxn, (int)hb.tlen[i]);
         if (table_ok && int16_fits(mx, dp.alg, dp.open, dp.gap, hb.qlen[i], hb.tlen[i])) {
             fast.push_back(i);
-            shape[i] = choose(16, hb.qlen[i], hb.tlen[i]);
+            shape[i] = choose(sw, 16, hb.qlen[i], hb.tlen[i]);
             total += shape[i].cost;
         } else if (nwlong_eligible(mx, dp.alg, dp.kind, dp.check, hb.qlen[i], hb.tlen[i])) {
             lng.push_back(i);
@@ -713,7 +725,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     const double pass = total / 8.0 / device_simds();
     for (int i : fast) {
         if (shape[i].cost > 0.25 * pass && hb.qlen[i] > 16 * 8) {
-            const Shape s64 = choose(64, hb.qlen[i], hb.tlen[i]);
+            const Shape s64 = choose(sw, 64, hb.qlen[i], hb.tlen[i]);
             shape[i] = s64;
             pT[i] = 64;
         }
@@ -1015,3 +1027,101 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream, hipEvent_
     }
     return 0;
 }
+
+// ------------------------------------------------------------ autotuning
+//
+// The planner's cost of a step with R rows per lane, measured: for each R a
+// batch whose every pair plans to one band of R rows per lane on 16-lane
+// chains (query 16 R rows, target 384 columns) and holds 8 waves per SIMD
+// runs on the wavefront kernel; its event-timed duration per wave-step is
+// R's cost.  SW and NW (SG runs NW's kernel) are measured apart.  Pairs
+// are random over the matrix's first 20 letters.  Calibration batches use
+// the instruction model, so that their shapes are the ones intended.
+extern "C" int phip_wave_autotune(const phip_matrix_t *mx, int open, int gap)
+{
+    if (phip_init() != 0) return -1;
+    if (!mx || mx->is_pssm || mx->size < 2 || open < 0 || gap < 0) {
+        phip_set_error("phip_wave_autotune: needs a square matrix and gaps >= 0");
+        return -1;
+    }
+    const int n = 384, T = 16;
+    const int waves = 8 * device_simds();
+    const int np = 8 * waves;  // 4 jobs of 2 pairs a wave
+    const int letters = std::min(20, mx->size - 1);
+    double cost[2][kRmax + 1] = {};
+    const bool was = g_tuned;
+    g_tuned = false;
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&]() -> uint8_t {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        return (uint8_t)(x % (uint64_t)letters);
+    };
+    std::vector<uint8_t> q, t((size_t)np * n);
+    for (uint8_t &c : t) c = rnd();
+    std::vector<int64_t> qoff(np), toff(np);
+    std::vector<int32_t> qlen(np), tlen(np, n);
+    std::vector<phip_out_t> out(np);
+    int rc = 0;
+    for (int sw = 0; sw < 2 && rc == 0; ++sw) {
+        for (int R : kRs) {
+            const int m = T * R;
+            q.resize((size_t)np * m);
+            for (uint8_t &c : q) c = rnd();
+            for (int i = 0; i < np; ++i) {
+                qoff[i] = (int64_t)i * m;
+                toff[i] = (int64_t)i * n;
+                qlen[i] = m;
+            }
+            phip_batch_t b;
+            memset(&b, 0, sizeof(b));
+            b.alg = sw ? PHIP_SW : PHIP_NW;
+            b.kind = PHIP_SCORE;
+            b.open = open;
+            b.gap = gap;
+            b.matrix = *mx;
+            b.npairs = np;
+            b.q = q.data();
+            b.qoff = qoff.data();
+            b.qlen = qlen.data();
+            b.t = t.data();
+            b.toff = toff.data();
+            b.tlen = tlen.data();
+            double best = 0;
+            for (int rep = 0; rep < 2 && rc == 0; ++rep) {
+                rc = phip_align(&b, out.data(), nullptr);
+                const double ms = phip_last_main_kernel_ms();
+                if (rep == 0 || ms < best) best = ms;
+            }
+            if (rc) break;
+            cost[sw][R] = best * 1e6 / ((double)waves * (n + T - 1));
+        }
+    }
+    if (rc) {
+        g_tuned = was;
+        return -1;
+    }
+    memcpy(g_step_cost, cost, sizeof(cost));
+    g_tuned = true;
+    return 0;
+}
+
+// The measured costs (ns per wave-step, device-wide) per rows-per-lane
+// choice, SW and NW/SG; the number of choices, 0 if untuned.
+extern "C" int phip_wave_autotune_table(int *rows, double *sw_cost, double *nw_cost, int cap)
+{
+    int k = 0;
+    for (int R : kRs) {
+        if (k < cap) {
+            if (rows) rows[k] = R;
+            if (sw_cost) sw_cost[k] = g_tuned ? g_step_cost[1][R] : 0;
+            if (nw_cost) nw_cost[k] = g_tuned ? g_step_cost[0][R] : 0;
+        }
+        ++k;
+    }
+    return g_tuned ? k : 0;
+}
+
+// Back to the instruction model.
+extern "C" void phip_wave_autotune_reset(void) { g_tuned = false; }

@@ -489,6 +489,33 @@ PHOST_EXPORT void parasail_hip_batch_free(parasail_hip_batch_t *hb)
     free(hb);
 }
 
+PHOST_EXPORT int parasail_hip_autotune(const parasail_matrix_t *matrix, int open, int gap)
+{
+    if (!matrix || matrix->type == PARASAIL_MATRIX_TYPE_PSSM || open < 0 || gap < 0) {
+        fprintf(stderr, "%s: invalid arguments\n", __func__);
+        return -1;
+    }
+    phip_matrix_t mx;
+    mx.values = matrix->matrix;
+    mx.size = matrix->size;
+    mx.length = matrix->size;
+    mx.is_pssm = 0;
+    mx.max = matrix->max;
+    mx.min = matrix->min;
+    if (phip_wave_autotune(&mx, open, gap) != 0) {
+        fprintf(stderr, "%s: %s\n", __func__, phip_last_error());
+        return -1;
+    }
+    return 0;
+}
+
+PHOST_EXPORT int parasail_hip_autotune_table(int *rows, double *sw_cost, double *nw_cost, int cap)
+{
+    return phip_wave_autotune_table(rows, sw_cost, nw_cost, cap);
+}
+
+PHOST_EXPORT void parasail_hip_autotune_reset(void) { phip_wave_autotune_reset(); }
+
 PHOST_EXPORT int parasail_hip_batch_set_graph(parasail_hip_batch_t *hb, int on)
 {
     return hb ? phip_resident_set_graph(hb->dev, on) : -1;

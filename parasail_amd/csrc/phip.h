@@ -160,6 +160,11 @@ void phip_nw_banded_last_stats(int *pairs, int *segments, int *redone);
 
 int phip_device_info(int *cus, int *clock_khz);
 
+/* measured step costs of the wavefront planner (see wave.hip) */
+int phip_wave_autotune(const phip_matrix_t *mx, int open, int gap);
+int phip_wave_autotune_table(int *rows, double *sw_cost, double *nw_cost, int cap);
+void phip_wave_autotune_reset(void);
+
 /* ---- database search and serving (search.hip) ---- */
 
 typedef struct phip_hit {

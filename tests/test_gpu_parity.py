@@ -809,6 +809,48 @@ def test_resident_batch_single_and_multi(mats):
         L.parasail_hip_batch_free(h)
 
 
+def test_autotuned_planner_same_results(mats):
+    """parasail_hip_autotune measures a step cost for every rows-per-lane
+    choice (SW and NW) and later plans use them: the shapes may change, the
+    results may not -- every pair equals its untuned result, which equals
+    the oracle on a sample."""
+    from parasail_amd import workloads as WL
+    pm, om = mats["blosum62"]
+    L = P.lib()
+    L.parasail_hip_autotune.restype = ctypes.c_int
+    L.parasail_hip_autotune.argtypes = [P.MP, ctypes.c_int, ctypes.c_int]
+    L.parasail_hip_autotune_table.restype = ctypes.c_int
+    L.parasail_hip_autotune_table.argtypes = [ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    b = WL.config1(seed=23, npairs=4000)
+    before = {name: P.align_arrays(name, 0, 0, b, 11, 1, pm) for name in ("sw", "nw", "sg")}
+    assert L.parasail_hip_autotune_table(None, None, None, 0) == 0
+    try:
+        assert L.parasail_hip_autotune(pm, 11, 1) == 0
+        rows = (ctypes.c_int * 16)()
+        swc, nwc = (ctypes.c_double * 16)(), (ctypes.c_double * 16)()
+        k = L.parasail_hip_autotune_table(rows, swc, nwc, 16)
+        assert k == 9 and [rows[i] for i in range(k)] == [4, 6, 8, 10, 12, 14, 16, 18, 20]
+        for i in range(k):
+            assert swc[i] > 0 and nwc[i] > 0, (i, swc[i], nwc[i])
+        # more rows per lane, more work per step
+        assert swc[k - 1] > swc[0] and nwc[k - 1] > nwc[0]
+        for name in ("sw", "nw", "sg"):
+            after = P.align_arrays(name, 0, 0, b, 11, 1, pm)
+            for key in ("score", "end_query", "end_ref", "flag"):
+                assert (after[key] == before[name][key]).all(), (name, key)
+    finally:
+        L.parasail_hip_autotune_reset()
+    assert L.parasail_hip_autotune_table(None, None, None, 0) == 0
+    for i in range(0, len(b), 400):
+        q = bytes(b.q[b.qoff[i]:b.qoff[i] + b.qlen[i]])
+        t = bytes(b.t[b.toff[i]:b.toff[i] + b.tlen[i]])
+        exp, _ = O.align(O.SW, O.SCORE, q, t, 11, 1, om)
+        assert (before["sw"]["score"][i], before["sw"]["end_query"][i],
+                before["sw"]["end_ref"][i]) == tuple(exp[:3]), i
+
+
 def test_resident_batch_graph_replays(mats):
     """A multi-pass call repeating the previous one is captured into a HIP
     graph and later ones replay it: every call's results equal the eager
