@@ -24,6 +24,11 @@ __device__ __forceinline__ int max(int a, int b)
     return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(s2v, a),
                                                             __builtin_bit_cast(s2v, b)));
 }
+__device__ __forceinline__ int min(int a, int b)
+{
+    return __builtin_bit_cast(int, __builtin_elementwise_min(__builtin_bit_cast(s2v, a),
+                                                            __builtin_bit_cast(s2v, b)));
+}
 // unsigned saturating subtraction of both halves (clamps at 0)
 __device__ __forceinline__ int subu(int a, int b)
 {
@@ -52,6 +57,7 @@ __device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_mov_dpp(v, 
 }  // namespace pk
 
 #define PK_NEG ((int)0x80008000)
+#define PK_POS ((int)0x7FFF7FFF)
 // substitution value of pad rows / columns: low enough that a pad cell
 // never competes with a real one, high enough that adding it to any int16
 // DP value saturates instead of wrapping

@@ -151,6 +151,11 @@ struct phip_resident {
     int64_t trtot;
     std::vector<WavePlan *> plans;
     bool last_wave = false;  // the last pass ran the wavefront tier
+    // a saturation check on NW/SG runs on the packed kernel when every pair
+    // fits it (cover, for cover_key's alg, open, gap) and min_h is not tested
+    bool want_min_h = false;
+    int cover_key[3] = {-1, -1, -1};
+    bool cover = false;
     // _stats / _trace: the wavefront plan (built for one check, alg, open,
     // gap), its device image, and per pair the trace layout of the last
     // pass (0 row-major, else dp32_layout)
@@ -195,6 +200,7 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
     const int np = b->npairs;
     r->np = np;
     r->kind = b->kind;
+    r->want_min_h = b->want_min_h != 0;
     r->has_q = b->q != nullptr;
     r->mx = b->matrix;
     r->mvals.assign(b->matrix.values, b->matrix.values + (size_t)b->matrix.size * b->matrix.length);
@@ -375,20 +381,31 @@ static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int
                         int check, hipEvent_t me0 = nullptr, hipEvent_t me1 = nullptr)
 {
     DPParams dp = make_params(r, alg, kind, sg_flags, open, gap, check);
-    const bool wave = kind == PHIP_SCORE && !r->mx.is_pssm && !(check && alg != PHIP_SW);
+    HostBatch hb;
+    hb.np = r->np;
+    hb.qlen = r->qlen.data();
+    hb.tlen = r->tlen.data();
+    hb.qoff = r->qoff.data();
+    hb.toff = r->toff.data();
+    hb.mx = &r->mx;
+    bool wave = kind == PHIP_SCORE && !r->mx.is_pssm;
+    if (wave && check && alg != PHIP_SW) {
+        // the packed kernel returns the extrema of H/E/F (not min_h); its
+        // int32 tiers inside the plan would leave long pairs to the general
+        // kernel, so a batch with such pairs takes the int32 lane chains
+        const int key[3] = {alg, open, gap};
+        if (memcmp(key, r->cover_key, sizeof(key)) != 0) {
+            r->cover = wave_covers(hb, dp);
+            memcpy(r->cover_key, key, sizeof(key));
+        }
+        wave = r->cover && !r->want_min_h;
+    }
     r->last_wave = wave;
     if (wave) {
         WavePlan *plan = nullptr;
         for (WavePlan *p : r->plans)
-            if (wave_plan_matches(p, alg, dp.sg_flags, open, gap)) plan = p;
+            if (wave_plan_matches(p, alg, dp.sg_flags, open, gap, check)) plan = p;
         if (!plan) {
-            HostBatch hb;
-            hb.np = r->np;
-            hb.qlen = r->qlen.data();
-            hb.tlen = r->tlen.data();
-            hb.qoff = r->qoff.data();
-            hb.toff = r->toff.data();
-            hb.mx = &r->mx;
             plan = wave_plan_create(hb, dp, g_stream);
             if (!plan) return -1;
             r->plans.push_back(plan);
@@ -400,13 +417,6 @@ static int resident_run(phip_resident_t *r, int alg, int kind, int sg_flags, int
         // tier for the rest
         const int key[4] = {check, alg, open, gap};
         if (memcmp(key, r->dplan_key, sizeof(key)) != 0) {
-            HostBatch hb;
-            hb.np = r->np;
-            hb.qlen = r->qlen.data();
-            hb.tlen = r->tlen.data();
-            hb.qoff = r->qoff.data();
-            hb.toff = r->toff.data();
-            hb.mx = &r->mx;
             std::vector<int> fast;
             std::vector<int32_t> status(r->np, PHIP_ST_W32);
             r->slow.clear();

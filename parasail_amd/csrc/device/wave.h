@@ -30,7 +30,10 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
 int wave_plan_run(WavePlan *p, const DPParams &dp, hipStream_t s, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);
 void wave_plan_free(WavePlan *p);
-bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap);
+bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap, int check);
+// Whether every pair of the batch goes to the packed-int16 kernel (none to
+// the int32 tiers).
+bool wave_covers(const HostBatch &hb, const DPParams &dp);
 // Device time (ms, HIP events on the launch stream) of the wavefront kernel
 // of the last wave_plan_run, once that stream has been synchronised; 0 if
 // the run launched none.

@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "dp32.h"
 #include "pk.h"
 #include "wave.h"
 
@@ -67,6 +68,7 @@ struct WaveJob {
 struct WaveDesc {
     int job0, njobs, R, T;
     int nb, width, bbw, prio;
+    int ext;  // NW/SG: some pair's saturation window needs the cells' extrema
 };
 
 struct WaveCfg {
@@ -98,8 +100,14 @@ struct PairRef {
 
 namespace {
 
-// All bands of one queue entry, R rows per lane.
-template <int R, bool SW>
+// All bands of one queue entry, R rows per lane.  EXT (NW/SG): also the
+// extrema the saturation windows test -- the largest H and the smallest E
+// and F (H is never below either) -- per lane over its pairs' columns.  The
+// virtual rows above row 0 change neither: their H and E repeat row 0 (H 0,
+// E -open when row 0 is free) and their F is row 1's (-open when free), so
+// their H is at most 0 and their E and F are values of the boundary row or
+// of real cells, which the reduction counts anyway.
+template <int R, bool SW, bool EXT>
 __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, const char *tabb,
                                          int lane, short *snp)
 {
@@ -119,6 +127,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
     // 0..m-2 (first row on ties) per half
     int cS[2] = {-1, -1}, cC[2] = {0, 0}, cR[2] = {0, 0};
     int cmx[2] = {PHIP_NEG_INF32, PHIP_NEG_INF32}, crow[2] = {0, 0};
+    int vmx = 0, vmn = 0;  // EXT: packed largest H, smallest E/F (from 0)
 
     for (int b = 0; b < wd.nb; ++b) {
         const bool band_live = live && b < J.nb;
@@ -175,6 +184,9 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         for (int i = 0; i < (SW ? 1 : R); ++i) snap[i] = 0;
         int bcolA = 0, bcolB = 0;
         const int lastA = J.n[0] - 1, lastB = J.n[1] - 1;
+        // EXT: the last column whose values count, per half (none past the
+        // job's bands, whose rows are not the pair's)
+        const int eA = band_live ? lastA : -1, eB = band_live ? lastB : -1;
 
         // column codes: ccl[k] is the code pair of column k - t; the stream
         // holds T leading and 8 trailing pad codes around the wave's width
@@ -218,6 +230,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 // SW passes H between lanes and bands, the others H - open
                 int up = SW ? pk::subu(rH, openv) : rH, F = rF;
                 int cm = 0;
+                int smx = 0, smn = 0;  // EXT: the column's largest H, smallest E/F
                 // each HL[i] (SW: Hs[i]) is read (E, next row's diagonal)
                 // before it is rewritten, so it stays in its register
 #pragma unroll
@@ -241,7 +254,18 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                         HL[i] = pk::sub(h, openv);
                         up = HL[i];
                         t1 = t1n;
+                        if (EXT) {
+                            const int ef = pk::min(E[i], F);
+                            smx = i == 0 ? h : pk::max(smx, h);
+                            smn = i == 0 ? ef : pk::min(smn, ef);
+                        }
                     }
+                }
+                if (EXT && !SW) {
+                    // a half past its target's last column keeps no value
+                    const int M = (j <= eA ? 0xffff : 0) | (j <= eB ? (int)0xffff0000 : 0);
+                    vmx = pk::max(vmx, (smx & M) | (PK_NEG & ~M));
+                    vmn = pk::min(vmn, (smn & M) | (PK_POS & ~M));
                 }
                 botH = SW ? Hs[R - 1] : HL[R - 1];
                 botF = F;
@@ -371,8 +395,28 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         for (int h = 0; h < 2; ++h) {
             if (J.pair[h] < 0) continue;
             if (SW) c.cand[J.cdoff[h] + t] = make_int4(cS[h], cC[h], cR[h], 0);
-            else c.cand[J.cdoff[h] + t] = make_int4(cmx[h], crow[h], 0, 0);
+            else
+                c.cand[J.cdoff[h] + t] = make_int4(cmx[h], crow[h], h ? pk::hi(vmn) : pk::lo(vmn),
+                                                   h ? pk::hi(vmx) : pk::lo(vmx));
         }
+    }
+}
+
+// One queue entry at its rows per lane.
+template <bool SW, bool EXT>
+__device__ __forceinline__ void wave_rows(const WaveCfg &c, const WaveDesc &wd, const char *tabb,
+                                          int lane, short *snp)
+{
+    switch (wd.R) {
+    case 4: wave_run<4, SW, EXT>(c, wd, tabb, lane, snp); break;
+    case 6: wave_run<6, SW, EXT>(c, wd, tabb, lane, snp); break;
+    case 8: wave_run<8, SW, EXT>(c, wd, tabb, lane, snp); break;
+    case 10: wave_run<10, SW, EXT>(c, wd, tabb, lane, snp); break;
+    case 12: wave_run<12, SW, EXT>(c, wd, tabb, lane, snp); break;
+    case 14: wave_run<14, SW, EXT>(c, wd, tabb, lane, snp); break;
+    case 16: wave_run<16, SW, EXT>(c, wd, tabb, lane, snp); break;
+    case 18: wave_run<18, SW, EXT>(c, wd, tabb, lane, snp); break;
+    default: wave_run<20, SW, EXT>(c, wd, tabb, lane, snp); break;
     }
 }
 
@@ -385,7 +429,7 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
 //
 // SW end snapshots follow the table in LDS: 2 x kSnapRows rows of 256
 // int16 slots, one per thread (half A rows, then half B rows).
-template <bool SW>
+template <bool SW, bool EXT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_wave(WaveCfg c)
 {
     extern __shared__ int tab32[];
@@ -403,17 +447,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         if (wv >= c.nwaves) break;
         const WaveDesc wd = c.waves[wv];
         if (wd.prio) __builtin_amdgcn_s_setprio(2);
-        switch (wd.R) {
-        case 4: wave_run<4, SW>(c, wd, tabb, lane, snp); break;
-        case 6: wave_run<6, SW>(c, wd, tabb, lane, snp); break;
-        case 8: wave_run<8, SW>(c, wd, tabb, lane, snp); break;
-        case 10: wave_run<10, SW>(c, wd, tabb, lane, snp); break;
-        case 12: wave_run<12, SW>(c, wd, tabb, lane, snp); break;
-        case 14: wave_run<14, SW>(c, wd, tabb, lane, snp); break;
-        case 16: wave_run<16, SW>(c, wd, tabb, lane, snp); break;
-        case 18: wave_run<18, SW>(c, wd, tabb, lane, snp); break;
-        default: wave_run<20, SW>(c, wd, tabb, lane, snp); break;
-    }
+        if (EXT && wd.ext) {
+            wave_rows<SW, true>(c, wd, tabb, lane, snp);
+        } else {
+            wave_rows<SW, false>(c, wd, tabb, lane, snp);
+        }
         if (wd.prio) __builtin_amdgcn_s_setprio(0);
     }
 }
@@ -472,7 +510,7 @@ __global__ void k_zero(int *p, int n)
 // free (SG with s2_end); otherwise the corner H[m-1][n-1] is all it needs.
 __global__ void __launch_bounds__(256) k_sg_reduce(const int4 *cand, const int2 *bandbuf,
                                                    const PairRef *refs, int n, int alg, int flags,
-                                                   int open, phip_out_t *out)
+                                                   int open, int gap, bool ext, phip_out_t *out)
 {
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -485,14 +523,28 @@ __global__ void __launch_bounds__(256) k_sg_reduce(const int4 *cand, const int2 
     const bool s2e = alg == PHIP_SG && (flags & PHIP_S2_END);
     // last-column candidates, per half
     int cm0 = PHIP_NEG_INF32, ci0 = 0, cm1 = PHIP_NEG_INF32, ci1 = 0;
-    if (s1e && lane < r0.T) {
+    // ext: the lanes' smallest E/F and largest H, per half (0 where no lane)
+    int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+    if ((s1e || ext) && lane < r0.T) {
         const int4 v = cand[r0.cdoff + lane];
-        cm0 = v.x;
+        cm0 = s1e ? v.x : PHIP_NEG_INF32;
         ci0 = v.y;
+        lo0 = v.z;
+        hi0 = v.w;
         if (two) {
             const int4 w = cand[r1.cdoff + lane];
-            cm1 = w.x;
+            cm1 = s1e ? w.x : PHIP_NEG_INF32;
             ci1 = w.y;
+            lo1 = w.z;
+            hi1 = w.w;
+        }
+    }
+    if (ext) {
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo0 = imin(lo0, __shfl_xor(lo0, d));
+            hi0 = imax(hi0, __shfl_xor(hi0, d));
+            lo1 = imin(lo1, __shfl_xor(lo1, d));
+            hi1 = imax(hi1, __shfl_xor(hi1, d));
         }
     }
     // last row: one pass over the shared band row serves both halves
@@ -543,6 +595,15 @@ __global__ void __launch_bounds__(256) k_sg_reduce(const int4 *cand, const int2 
     o.score = score;
     o.end_query = eq;
     o.end_ref = er;
+    if (ext) {
+        // the cells' extrema with the DP's boundary row and column
+        // (-open - (len-1) gap at their ends unless that gap is free)
+        int vmin = lane ? lo1 : lo0;
+        if (!(alg == PHIP_SG && (flags & PHIP_S2_BEG))) vmin = imin(vmin, -open - (nn - 1) * gap);
+        if (!(alg == PHIP_SG && (flags & PHIP_S1_BEG))) vmin = imin(vmin, -open - (m - 1) * gap);
+        o.min_val = vmin;
+        o.max_val = lane ? hi1 : hi0;
+    }
     out[r.pair] = o;
 }
 
@@ -630,9 +691,31 @@ bool int16_fits(const phip_matrix_t &mx, int alg, int open, int gap, int m, int 
     if (mx.max + open > 16000 || mx.min < -16000 || open > 8000 || gap > 8000) return false;
     if (m > 30000 || n > 30000) return false;
     if (alg == PHIP_SW) return true;
+    // the virtual rows above a query repeat row 0 (H -open - (j-1) gap) by
+    // horizontal gaps from H = 0, which reopen more cheaply than they
+    // extend when gap > open
+    if (gap > open) return false;
     int64_t lo = -(int64_t)4 * open - (int64_t)(m + n) * gap - std::abs(mx.min) - 64;
     int64_t hi = (int64_t)std::min(m, n) * std::max(mx.max, 0) + mx.max + open + 64;
     return lo > -32000 && hi < 32000;
+}
+
+// NW/SG with a saturation check: whether the pair's window at dp.check is
+// decided without the cells' extrema -- it provably holds, or the DP's
+// boundary row or column alone leaves it and the next window (16 bits;
+// after 16, 32) provably holds.  The reduction reports the boundary's
+// extrema, and the cells' only when some pair of the plan is not settled.
+bool window_settled(const phip_matrix_t &mx, const DPParams &dp, int m, int n)
+{
+    if (dp32_window_holds(mx, dp.alg, PHIP_SCORE, dp.check, dp.open, dp.gap, m, n)) return true;
+    const int64_t wmin = -((int64_t)1 << (dp.check - 1));
+    const int64_t neg = (-dp.open < mx.min ? wmin + dp.open : wmin - (int64_t)mx.min) + 1;
+    const bool s1_beg = dp.alg == PHIP_SG && (dp.sg_flags & PHIP_S1_BEG);
+    const bool s2_beg = dp.alg == PHIP_SG && (dp.sg_flags & PHIP_S2_BEG);
+    const int64_t top = s2_beg ? 0 : -(int64_t)dp.open - (int64_t)(n - 1) * dp.gap;
+    const int64_t left = s1_beg ? 0 : -(int64_t)dp.open - (int64_t)(m - 1) * dp.gap;
+    return std::min(top, left) < neg &&
+           (dp.check >= 16 || dp32_window_holds(mx, dp.alg, PHIP_SCORE, 16, dp.open, dp.gap, m, n));
 }
 
 size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -655,6 +738,8 @@ int device_simds()
 
 struct WavePlan {
     int alg, sg_flags, open, gap;
+    int check;    // saturation window whose extrema the results carry (0: none)
+    bool ext;     // NW/SG: some pair's window needs the cells' extrema
     bool sw;
     int nwaves = 0, nblocks = 0;
     int nfast = 0, nslow = 0;
@@ -675,9 +760,20 @@ struct WavePlan {
     char *dlong = nullptr;
 };
 
-bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap)
+bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap, int check)
 {
-    return p->alg == alg && p->sg_flags == sg_flags && p->open == open && p->gap == gap;
+    return p->alg == alg && p->sg_flags == sg_flags && p->open == open && p->gap == gap &&
+           p->check == check;
+}
+
+bool wave_covers(const HostBatch &hb, const DPParams &dp)
+{
+    const phip_matrix_t &mx = *hb.mx;
+    const int tsp0 = (mx.size + 2) & ~1;
+    if ((size_t)(mx.size + 2) * (tsp0 / 2) * 4 > 65535u) return false;
+    for (int i = 0; i < hb.np; ++i)
+        if (!int16_fits(mx, dp.alg, dp.open, dp.gap, hb.qlen[i], hb.tlen[i])) return false;
+    return true;
 }
 
 void wave_plan_free(WavePlan *p)
@@ -694,6 +790,8 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     P->sg_flags = dp.sg_flags;
     P->open = dp.open;
     P->gap = dp.gap;
+    P->check = dp.check;
+    P->ext = false;
     P->sw = dp.alg == PHIP_SW;
     const bool sw = P->sw;
     const phip_matrix_t &mx = *hb.mx;
@@ -701,6 +799,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     std::vector<int> fast, slow;
     std::vector<Shape> shape(np);
     std::vector<int> pT(np, 16);
+    std::vector<char> unsettled(np, 0);
     // LDS table geometry (below); its byte offsets must fit the 16-bit
     // halves of the packed row offsets, or every pair takes the int32 tier
     const int tsp0 = (mx.size + 2) & ~1;
@@ -711,6 +810,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
         P->maxn = std::max(P->maxn, (int)hb.tlen[i]);
         if (table_ok && int16_fits(mx, dp.alg, dp.open, dp.gap, hb.qlen[i], hb.tlen[i])) {
             fast.push_back(i);
+            if (!sw && dp.check && !window_settled(mx, dp, hb.qlen[i], hb.tlen[i])) unsettled[i] = 1;
             shape[i] = choose(sw, 16, hb.qlen[i], hb.tlen[i]);
             total += shape[i].cost;
         } else if (nwlong_eligible(mx, dp.alg, dp.kind, dp.check, hb.qlen[i], hb.tlen[i])) {
@@ -773,6 +873,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     // job (A low, B high); then entries of 64/T jobs
     std::vector<WaveJob> jobs;
     std::vector<int> jT, jR;
+    std::vector<char> jext;
     std::vector<double> jcost;
     size_t k = 0;
     while (k < fast.size()) {
@@ -789,8 +890,10 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
         J.pair[1] = -1;
         J.cdoff[1] = -1;
         double cost = 0;
+        char ext = 0;
         for (int h = 0; h < halves; ++h) {
             const int p = fast[k + h];
+            ext |= unsettled[p];
             J.pair[h] = p;
             J.m[h] = hb.qlen[p];
             J.n[h] = hb.tlen[p];
@@ -801,6 +904,7 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
         jobs.push_back(J);
         jT.push_back(pT[p0]);
         jR.push_back(shape[p0].R);
+        jext.push_back(ext);
         jcost.push_back(cost);
         k += halves;
     }
@@ -821,8 +925,10 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
                 w.nb = std::max(w.nb, jobs[j1].nb);
                 w.width = std::max(w.width, std::max(jobs[j1].n[0], jobs[j1].n[1]));
                 cst = std::max(cst, jcost[j1]);
+                w.ext |= jext[j1];
                 ++j1;
             }
+            P->ext = P->ext || w.ext;
             w.njobs = (int)(j1 - j0);
             w.bbw = (w.width + T + 8 + 15) & ~15;
             w.prio = T == 64;
@@ -992,7 +1098,7 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream, hipEvent_
         cfg.gap = P->gap;
         cfg.alg = P->alg;
         cfg.sg_flags = P->sg_flags;
-        KFn fn = P->sw ? k_wave<true> : k_wave<false>;
+        KFn fn = P->sw ? k_wave<true, false> : P->ext ? k_wave<false, true> : k_wave<false, false>;
         HIP_OK(hipEventRecord(g_kw0, stream));
         if (e0) HIP_OK(hipEventRecordWithFlags(e0, stream, hipEventRecordExternal));
         const size_t lds = (size_t)((cfg.tabn + 3) & ~3) * 4 + (P->sw ? 2 * kSnapRows * 256 * 2 : 0);
@@ -1007,7 +1113,8 @@ int wave_plan_run(WavePlan *P, const DPParams &dp, hipStream_t stream, hipEvent_
                                P->dcount);
         } else {
             hipLaunchKernelGGL(k_sg_reduce, dim3((P->nfast + 3) / 4), dim3(256), 0, stream, P->dcd,
-                               P->dbb, P->drefs, P->nfast, P->alg, P->sg_flags, P->open, dp.out);
+                               P->dbb, P->drefs, P->nfast, P->alg, P->sg_flags, P->open, P->gap,
+                               P->check != 0, dp.out);
         }
         HIP_OK(hipGetLastError());
     }

@@ -175,6 +175,7 @@ static parasail_result_t *align_ex(const char *fname, int alg, int sg_flags, int
     b.gap = gap;
     /* the narrowest width whose saturation the result must report */
     b.want_sat_check = phost_check_width(alg, kind, scan, width);
+    b.want_min_h = kind == PHIP_STATS || (scan && kind == PHIP_SCORE && alg == PHIP_NW);
     b.matrix.values = matrix->matrix;
     b.matrix.size = matrix->size;
     b.matrix.length = pssm ? matrix->length : matrix->size;

@@ -64,16 +64,10 @@ static void convert_range(void *ctx, int lo, int hi)
         int w = width, sat = 0;
         if (w == 0) {
             /* the narrowest width whose window holds (satcheck.c tries 8,
-             * then 16, then 32), from the exact score (SW) or the exact
-             * extrema the device returned (_stats, _trace); NW/SG score-only
-             * batches report 16 unless an int32 tier ran, since their 8-bit
-             * window needs the table's extrema, which the packed-int16
-             * kernel does not compute */
-            if (alg == PHIP_SW || kind != PHIP_SCORE)
-                w = !phost_saturates(alg, kind, 0, 8, open, matrix, o) ? 8
-                    : !phost_saturates(alg, kind, 0, 16, open, matrix, o) ? 16 : 32;
-            else
-                w = (o->status & PHIP_ST_W32) ? 32 : 16;
+             * then 16, then 32), from the exact score (SW) or the extrema
+             * the device returned (NW/SG, _stats) */
+            w = !phost_saturates(alg, kind, 0, 8, open, matrix, o) ? 8
+                : !phost_saturates(alg, kind, 0, 16, open, matrix, o) ? 16 : 32;
         } else {
             sat = phost_saturates(alg, kind, 0, w, open, matrix, o);
         }
@@ -237,10 +231,9 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
         b.sg_flags = flags;
         b.open = open;
         b.gap = gap;
-        /* the extrema the flags need: none for NW/SG score-only "sat"
-         * batches (see convert_range) */
-        b.want_sat_check = width == 0 && alg != PHIP_SW && kind == PHIP_SCORE
-                               ? 0 : phost_check_width(alg, kind, 0, width);
+        /* the extrema the flags need */
+        b.want_sat_check = phost_check_width(alg, kind, 0, width);
+        b.want_min_h = kind == PHIP_STATS;
         b.matrix.values = matrix->matrix;
         b.matrix.size = matrix->size;
         b.matrix.length = pssm ? matrix->length : matrix->size;

@@ -45,6 +45,8 @@ typedef struct phip_batch {
     int open, gap;
     int want_sat_check;   /* 0, or the width (8, 16) whose saturation the
                            * caller will test: exact extrema are returned */
+    int want_min_h;       /* with want_sat_check: min_h is tested too (NW
+                           * scan score), which the packed kernel omits */
     phip_matrix_t matrix;
     int npairs;
     /* mapped residue codes (matrix->mapper), host memory.  For a PSSM the

@@ -777,8 +777,9 @@ def test_resident_batch_single_and_multi(mats):
             arrs, br = _bufs(n)
             assert L.parasail_hip_batch_align(h, name.encode(), 11, 1, ctypes.byref(br)) == 0
             ref = P.align_arrays(name, 0, 0, b, 11, 1, pm)
-            for key in ("score", "end_query", "end_ref", "flag"):
+            for key in ("score", "end_query", "end_ref"):
                 assert (arrs[key] == ref[key]).all(), (name, key)
+            _resident_flags_agree(name, arrs["flag"], ref["flag"])
             single[name] = arrs
         # spot check against the oracle
         for i in range(0, n, 300):
@@ -1067,6 +1068,84 @@ def test_batch_stats_sat_widths_follow_counts(mats):
                         (one["length"], one["matches"])
 
 
+def test_batch_sat_nw_sg_reports_narrowest_width(mats):
+    """A `_sat` NW/SG batch reports, pair by pair, the width of the per-call
+    `_sat` function (src/satcheck.c tries 8, then 16): 8 where every H/E/F
+    of the table and its boundary stays in the 8-bit window (the reference's
+    nw/sg_striped_*_8 test, sg_striped_avx2_256_8.c:317-320), which the
+    oracle's table decides here for the short pairs.  The batch's short
+    pairs run on the packed kernel's extrema variant, the long ones are
+    settled by their boundary.  A fixed-width 8 batch saturates the same
+    pairs as the per-call `_8` function."""
+    rng = random.Random(91)
+    pm, om = mats["blosum62"]
+    M = om.values()
+    qs, ts = [], []
+    for k in range(48):
+        a = rseq(rng, rng.randint(8, 100), weights=RR)
+        b = mutate(rng, a, AA, sub=0.2, indel=0.03) if k % 3 == 0 else rseq(rng, rng.randint(8, 100))
+        qs.append(a)
+        ts.append(b)
+    qs += ["W" * 12, "W" * 9, rseq(rng, 300), rseq(rng, 104), rseq(rng, 40)]
+    ts += ["W" * 12, "W" * 9, rseq(rng, 280), rseq(rng, 30), rseq(rng, 110)]
+    bits = P.FLAG_BITS_8 | P.FLAG_BITS_16 | P.FLAG_BITS_32
+    for name in ("nw", "sg", "sg_qx", "sg_db"):
+        alg, flags = oracle_alg(name)
+        got = P.align_batch(name, 0, 0, qs, ts, 11, 1, pm)
+        got8 = P.align_batch(name, 0, 8, qs, ts, 11, 1, pm)
+        exp, _ = O.batch(alg, O.SCORE, [om.map(q) for q in qs], [om.map(t) for t in ts],
+                         11, 1, om, flags)
+        widths = set()
+        for k, (q, t) in enumerate(zip(qs, ts)):
+            ctx = (name, k, len(q), len(t))
+            one = P.align("parasail_%s_striped_hip_64_sat" % name, q, t, 11, 1, pm)
+            assert int(got["score"][k]) == one["score"] == exp[k, 0], ctx
+            assert int(got["flag"][k]) & bits == one["flag"] & bits, ctx
+            if max(len(q), len(t)) <= 120:
+                vals = _dp_values(alg, flags, om.map(q), om.map(t), M, 11, 1)
+                sat8 = _ref_saturates(alg, "score", False, 8, 11, 11, -4, exp[k, 0], vals)
+                want = P.FLAG_BITS_16 if sat8 else P.FLAG_BITS_8
+                assert int(got["flag"][k]) & bits == want, ctx
+            one8 = P.align("parasail_%s_striped_hip_64_8" % name, q, t, 11, 1, pm)
+            assert int(got8["flag"][k]) == one8["flag"], ctx
+            assert int(got8["score"][k]) == one8["score"], ctx
+            widths.add(int(got["flag"][k]) & bits)
+        assert widths == {P.FLAG_BITS_8, P.FLAG_BITS_16}, name
+
+
+def test_batch_gap_above_open(mats):
+    """Extension dearer than opening (open 2, extend 3): NW and SG batches
+    equal the oracle (the global boundary row -open - (j-1) gap is then not
+    the cheapest run of gaps from the corner)."""
+    rng = random.Random(92)
+    pm, om = mats["blosum62"]
+    qs = [rseq(rng, rng.randint(1, 400), weights=RR) for _ in range(120)]
+    ts = [rseq(rng, rng.randint(1, 400), weights=RR) for _ in range(120)]
+    for name in ("nw", "sg", "sg_qb", "sg_db"):
+        alg, flags = oracle_alg(name)
+        got = P.align_batch(name, 0, 0, qs, ts, 2, 3, pm)
+        exp, _ = O.batch(alg, O.SCORE, [om.map(q) for q in qs], [om.map(t) for t in ts],
+                         2, 3, om, flags)
+        for k in range(len(qs)):
+            assert (int(got["score"][k]), int(got["end_query"][k]), int(got["end_ref"][k])) == \
+                tuple(int(v) for v in exp[k, :3]), (name, k, len(qs[k]), len(ts[k]))
+
+
+def _resident_flags_agree(name, res, batch):
+    """The resident batch's flags against parasail_hip_align_batch's at
+    width sat: the same for SW; for NW/SG the same but for the width, which
+    the resident batch reports as its tier's (16, or 32 for the int32 tier;
+    never below the sat function's)."""
+    bits = P.FLAG_BITS_8 | P.FLAG_BITS_16 | P.FLAG_BITS_32
+    if name == "sw":
+        assert (res == batch).all()
+        return
+    assert ((res & ~bits) == (batch & ~bits)).all(), name
+    rb, bb = res & bits, batch & bits
+    assert np.isin(rb, (P.FLAG_BITS_16, P.FLAG_BITS_32)).all(), name
+    assert (rb[bb == P.FLAG_BITS_32] == P.FLAG_BITS_32).all(), name
+
+
 def _concat(seqs):
     bs = [s.encode() for s in seqs]
     ln = np.array([len(x) for x in bs], dtype=np.int32)
@@ -1084,9 +1163,10 @@ def test_batch_int16_overflow_redo(mats):
     src/sw_striped_avx2_256_16.c:350-362; the 300 one fits the biased 16-bit
     window (2^16 - max - 2), the 700 one only 32 bits, and width 16 reports
     it saturated (src/sw_striped_avx2_256_16.c:374-382).  NW/SG: a pair past
-    the provable int16 bound runs on an int32 tier (its sat flag reads 32).
-    Scores and ends equal the oracle through parasail_hip_align_batch and
-    the resident batch; the SW sat flags are the per-call sat function's."""
+    the provable int16 bound runs on an int32 tier (the resident batch's
+    flag reads 32).  Scores and ends equal the oracle through
+    parasail_hip_align_batch and the resident batch; the batch's sat flags
+    are the per-call sat function's."""
     rng = random.Random(90)
     pb, ob = mats["blosum62"]
     alpha = "ACDEFGHIKLMNPQRSTVWY"
@@ -1127,16 +1207,24 @@ def test_batch_int16_overflow_redo(mats):
             assert L.parasail_hip_batch_align(h, name.encode(), 11, gap, ctypes.byref(br)) == 0
         finally:
             L.parasail_hip_batch_free(h)
-        for key in ("score", "end_query", "end_ref", "flag"):
+        bits = P.FLAG_BITS_8 | P.FLAG_BITS_16 | P.FLAG_BITS_32
+        for key in ("score", "end_query", "end_ref") + (("flag",) if name == "sw" else ()):
             assert (arrs[key] == got[key]).all(), (name, key)
         for k in range(len(qn)):
             exp, _ = O.align(alg, O.SCORE, qn[k], tn[k], 11, gap, om, fl)
             assert (got["score"][k], got["end_query"][k], got["end_ref"][k]) == exp[:3], \
                 (name, k, len(qn[k]), len(tn[k]))
         if name != "sw":
-            assert got["flag"][260] & P.FLAG_BITS_32
+            # the resident batch reports its tier's width (int32 for the
+            # long pair); the batch the per-call sat function's, whose
+            # windows follow the table's values
+            res_w = np.where(np.arange(len(qn)) == 260, P.FLAG_BITS_32, P.FLAG_BITS_16)
+            assert ((arrs["flag"] & bits) == res_w).all()
+            assert ((arrs["flag"] & ~bits) == (got["flag"] & ~bits)).all()
+            for k in (0, 1, 2, 260):
+                one = P.align("parasail_%s_striped_hip_64_sat" % name, qn[k], tn[k], 11, gap, pm)
+                assert int(got["flag"][k]) & bits == one["flag"] & bits, (name, k)
             continue
-        bits = P.FLAG_BITS_8 | P.FLAG_BITS_16 | P.FLAG_BITS_32
         ovf = 0x7BFF - (120 + 11 + 1)
         assert ovf < got["score"][40] <= 65535 - 122 < got["score"][170]
         assert got["score"][80] == 263 * 120 <= ovf < 265 * 120 == got["score"][120]
