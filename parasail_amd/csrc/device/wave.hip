@@ -51,6 +51,15 @@ __host__ __device__ __forceinline__ int clamp16(int v) { return v < -32768 ? -32
 
 // largest rows per lane (the SW snapshot area in LDS is sized for it)
 constexpr int kSnapRows = 20;
+// rows between a row's table reads and its use (2 kLA - 2 reads in flight
+// behind each: at most 15 can be waited on exactly)
+constexpr int kLA = 8;
+
+// two consecutive column codes of a job's stream, read as one 16-byte load
+// (at any 8-byte boundary)
+struct __attribute__((aligned(8))) CodePair {
+    unsigned x, y, z, w;
+};
 
 // One job (two pairs).  Host-built, device-read.
 struct WaveJob {
@@ -111,6 +120,7 @@ template <int R, bool SW, bool EXT>
 __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, const char *tabb,
                                          int lane, short *snp)
 {
+    constexpr int LA = kLA < R ? kLA : R;  // read-ahead distance in rows
     const int T = wd.T;
     const int t = lane & (T - 1);
     const int jslot = T == 64 ? 0 : lane >> 4;
@@ -198,34 +208,51 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         int2 *const bbr = c.bandbuf + J.bboff + (int64_t)b * wd.bbw;
 
         // the rows' scores against one column: W + open (SW: W) of both
-        // halves from the LDS table, at the code pair's byte offsets
-        int W[R];
-        // (the dword holding each half's entry is read whole: a 32-bit load
-        // result needs no masking wherever the compiler places the select)
-        auto lookup = [&](const int i, const uint2 code) __attribute__((always_inline)) {
-            const unsigned wa = *(const unsigned *)(tabb + (qo[i] & 0xffff) + (code.x & 0xffff));
-            const unsigned wb = *(const unsigned *)(tabb + ((unsigned)qo[i] >> 16) + (code.x >> 16));
-            W[i] = (int)__builtin_amdgcn_perm(wb, wa, code.y);
+        // halves from the LDS table, at the code pair's byte offsets.  Row
+        // k's two table dwords are read LA rows before row k uses them (for
+        // the rows below LA, during the previous column's last rows), so at
+        // most 2 LA reads are in flight and each use waits for its own
+        // reads only; the v_perm that takes each half's entry runs at the use
+        unsigned wa[R], wb[R];
+        auto issue = [&](const int k, const uint2 code) __attribute__((always_inline)) {
+            wa[k] = *(const unsigned *)(tabb + (qo[k] & 0xffff) + (code.x & 0xffff));
+            wb[k] = *(const unsigned *)(tabb + ((unsigned)qo[k] >> 16) + (code.x >> 16));
+        };
+        // after row k's entries are used: the read LA rows further down
+        // (this column's, or the next column's top rows)
+        auto refill = [&](const int k, const uint2 cur, const uint2 nxt) __attribute__((always_inline)) {
+            if (k + LA < R) issue(k + LA, cur);
+            else issue(k + LA - R, nxt);
         };
 
-        // one DP column.  W holds this column's scores, looked up during
-        // the previous step: each row's entry for the next column is read
-        // from LDS as soon as this column has used it, so the reads are in
-        // flight while the rest of the column is computed.  cn holds the
-        // next column's code pair, fetched two steps earlier, and is
-        // refilled three columns ahead after its last use; bx, the row
+        // one DP column.  p holds this column's code pair (x, y) and the
+        // next column's (z, w), fetched two steps earlier; bx, the row
         // above for lane 0, is fetched two steps ahead the same way.
         // FAST: every lane's column lies in [0, width) (the steady state)
-        auto step = [&](auto fastc, const int sv, uint2 &cn, int2 &bx) __attribute__((always_inline)) {
+        auto step = [&](auto fastc, const int sv, CodePair &p, int2 &bx) __attribute__((always_inline)) {
             constexpr bool FAST = decltype(fastc)::value;
             const int j = sv - t;
+            const uint2 cc = make_uint2(p.x, p.y), cn = make_uint2(p.z, p.w);
+            const int2 bxs = bx;
+            // this step's codes and row above (loaded two steps ago) are
+            // waited for before the loads for two steps ahead are issued:
+            // vector loads may return out of order, so the wait for one is
+            // a wait for every load in flight (vmcnt(0)), which would
+            // otherwise include the ones just issued
+            asm volatile("" ::"v"(cc.x), "v"(cc.y), "v"(cn.x), "v"(bxs.x), "v"(bxs.y));
+            __builtin_amdgcn_sched_barrier(0);
+            p = *(const CodePair *)(ccl + sv + 2);
+            bx = bbs[sv + 2];
+            auto wval = [&](const int k) __attribute__((always_inline)) {
+                return (int)__builtin_amdgcn_perm(wb[k], wa[k], cc.y);
+            };
             const int xH = pk::shr1(botH);  // every lane enabled
             const int xF = pk::shr1(botF);
             if (FAST || (j >= 0 && j < width)) {
-                const int rH = t == 0 ? bx.x : xH;
-                const int rF = t == 0 ? bx.y : xF;
-                int t1 = pk::add(prevRH, W[0]);
-                lookup(0, cn);
+                const int rH = t == 0 ? bxs.x : xH;
+                const int rF = t == 0 ? bxs.y : xF;
+                int t1 = pk::add(prevRH, wval(0));
+                refill(0, cc, cn);
                 prevRH = rH;
                 // SW passes H between lanes and bands, the others H - open
                 int up = SW ? pk::subu(rH, openv) : rH, F = rF;
@@ -237,8 +264,8 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                 for (int i = 0; i < R; ++i) {
                     if (SW) {
                         E[i] = pk::max(pk::subu(E[i], gapv), HL[i]);
-                        const int t1n = i + 1 < R ? pk::add(Hs[i], W[i + 1]) : 0;
-                        if (i + 1 < R) lookup(i + 1, cn);
+                        const int t1n = i + 1 < R ? pk::add(Hs[i], wval(i + 1)) : 0;
+                        if (i + 1 < R) refill(i + 1, cc, cn);
                         F = pk::max(pk::subu(F, gapv), up);
                         const int h = pk::max(pk::max(t1, E[i]), F);
                         Hs[i] = h;
@@ -247,8 +274,8 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                         t1 = t1n;
                     } else {
                         E[i] = pk::max(pk::sub(E[i], gapv), HL[i]);
-                        const int t1n = i + 1 < R ? pk::add(HL[i], W[i + 1]) : 0;
-                        if (i + 1 < R) lookup(i + 1, cn);
+                        const int t1n = i + 1 < R ? pk::add(HL[i], wval(i + 1)) : 0;
+                        if (i + 1 < R) refill(i + 1, cc, cn);
                         F = pk::max(pk::sub(F, gapv), up);
                         const int h = pk::max(pk::max(t1, E[i]), F);
                         HL[i] = pk::sub(h, openv);
@@ -320,19 +347,17 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
                     }
                 }
             } else {
+                // a lane outside the band's columns reads the next column's
+                // top rows, which it needs if that column is its first
 #pragma unroll
-                for (int i = 0; i < R; ++i) lookup(i, cn);
+                for (int k = 0; k < LA; ++k) issue(k, cn);
             }
-            cn = ccl[sv + 3];
-            bx = bbs[sv + 2];
         };
-        // column codes: c1 holds odd columns' and c0 even columns' (the
-        // step at sv looks up column sv + 1)
-        uint2 c0 = ccl[0];
+        // code pairs: p0 for even steps, p1 for odd ones; step sv works on
+        // column sv (relative to the lane) and reads ahead into sv + 1
 #pragma unroll
-        for (int i = 0; i < R; ++i) lookup(i, c0);
-        c0 = ccl[2];
-        uint2 c1 = ccl[1];
+        for (int k = 0; k < LA; ++k) issue(k, ccl[0]);
+        CodePair p0 = *(const CodePair *)(ccl + 0), p1 = *(const CodePair *)(ccl + 1);
         int2 b0 = bbs[0], b1 = bbs[1];
         const std::integral_constant<bool, true> fastc;
         const std::integral_constant<bool, false> slowc;
@@ -341,16 +366,16 @@ __device__ __forceinline__ void wave_run(const WaveCfg &c, const WaveDesc &wd, c
         const int lo = min(T, steps), hi = max(lo, width - 1);
         int s = 0;
         for (; s < lo; s += 2) {
-            step(slowc, s, c1, b0);
-            step(slowc, s + 1, c0, b1);
+            step(slowc, s, p0, b0);
+            step(slowc, s + 1, p1, b1);
         }
         for (; s < hi; s += 2) {
-            step(fastc, s, c1, b0);
-            step(fastc, s + 1, c0, b1);
+            step(fastc, s, p0, b0);
+            step(fastc, s + 1, p1, b1);
         }
         for (; s < steps; s += 2) {
-            step(slowc, s, c1, b0);
-            step(slowc, s + 1, c0, b1);
+            step(slowc, s, p0, b0);
+            step(slowc, s + 1, p1, b1);
         }
         if (SW && band_live) {
 #pragma unroll
