@@ -12,6 +12,7 @@
 #include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -571,6 +572,15 @@ extern "C" int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **trace
 
 // ---------------------------------------------------------------- one pair
 
+// A profile's device copy: one allocation holding the query codes, then the
+// matrix values at a 256-byte boundary.  It is made with hipMalloc, not the
+// pool, since it lives as long as the caller's profile.
+struct phip_dprofile {
+    uint8_t *q;
+    int *mat;
+    int m;
+};
+
 namespace {
 
 // pinned staging and its device mirror for per-call alignments (kept for
@@ -626,7 +636,7 @@ bool same_scoring(const phip_batch_t *a, const phip_batch_t *b)
     return a->alg == b->alg && a->kind == b->kind && a->sg_flags == b->sg_flags &&
            a->open == b->open && a->gap == b->gap && a->want_sat_check == b->want_sat_check &&
            a->matrix.values == b->matrix.values && a->matrix.size == b->matrix.size &&
-           a->matrix.is_pssm == b->matrix.is_pssm;
+           a->matrix.is_pssm == b->matrix.is_pssm && a->dprof == b->dprof;
 }
 
 // one launch for the requests of one scoring; each request's rc is set
@@ -635,6 +645,8 @@ void one_batch(const std::vector<OneReq *> &reqs)
     const int np = (int)reqs.size();
     const phip_batch_t *b0 = reqs[0]->b;
     const int kind = b0->kind, msize = b0->matrix.size;
+    // a profile's calls share its device query and matrix (same_scoring)
+    const phip_dprofile_t *dpf = b0->dprof;
     std::vector<int32_t> ql(np), tl(np), status(np, PHIP_ST_W32);
     std::vector<int64_t> qo(np), to(np), tro(np, 0);
     int64_t qtot = 0, ttot = 0, trtot = 0;
@@ -643,9 +655,9 @@ void one_batch(const std::vector<OneReq *> &reqs)
         const phip_batch_t *b = reqs[k]->b;
         ql[k] = b->qlen[0];
         tl[k] = b->tlen[0];
-        qo[k] = qtot;
+        qo[k] = dpf ? 0 : qtot;
         to[k] = ttot;
-        qtot += ql[k];
+        if (!dpf) qtot += ql[k];
         ttot += tl[k];
         if (kind == PHIP_TRACE) {
             tro[k] = trtot;
@@ -675,7 +687,7 @@ void one_batch(const std::vector<OneReq *> &reqs)
         auto slot = [&](size_t bytes) { size_t o = off; off += al256(bytes ? bytes : 1); return o; };
         const size_t o_plan = slot(P.bytes), o_out = slot(sizeof(phip_out_t) * np),
                      o_qo = slot(8 * np), o_to = slot(8 * np), o_ql = slot(4 * np),
-                     o_tl = slot(4 * np), o_mat = slot((size_t)msize * msize * 4),
+                     o_tl = slot(4 * np), o_mat = slot(dpf ? 0 : (size_t)msize * msize * 4),
                      o_q = slot((size_t)qtot), o_t = slot((size_t)ttot);
         const size_t up_end = off;
         const size_t o_tr = slot((size_t)trtot);
@@ -688,9 +700,9 @@ void one_batch(const std::vector<OneReq *> &reqs)
             memcpy(h + o_to, to.data(), 8 * np);
             memcpy(h + o_ql, ql.data(), 4 * np);
             memcpy(h + o_tl, tl.data(), 4 * np);
-            memcpy(h + o_mat, b0->matrix.values, (size_t)msize * msize * 4);
+            if (!dpf) memcpy(h + o_mat, b0->matrix.values, (size_t)msize * msize * 4);
             for (int k = 0; k < np; ++k) {
-                memcpy(h + o_q + qo[k], reqs[k]->b->q, (size_t)ql[k]);
+                if (!dpf) memcpy(h + o_q + qo[k], reqs[k]->b->q, (size_t)ql[k]);
                 memcpy(h + o_t + to[k], reqs[k]->b->t, (size_t)tl[k]);
             }
             if (hipMemcpyAsync(d, h, up_end, hipMemcpyHostToDevice, g_stream) != hipSuccess) {
@@ -705,9 +717,9 @@ void one_batch(const std::vector<OneReq *> &reqs)
             dp.open = b0->open;
             dp.gap = b0->gap;
             dp.check = ext ? b0->want_sat_check : 0;
-            dp.matrix = (const int *)(d + o_mat);
+            dp.matrix = dpf ? dpf->mat : (const int *)(d + o_mat);
             dp.msize = msize;
-            dp.q = (const uint8_t *)(d + o_q);
+            dp.q = dpf ? dpf->q : (const uint8_t *)(d + o_q);
             dp.qoff = (const int64_t *)(d + o_qo);
             dp.qlen = (const int32_t *)(d + o_ql);
             dp.t = (const uint8_t *)(d + o_t);
@@ -758,6 +770,11 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
     if (!b || !out || b->npairs != 1 || !b->q || (b->kind == PHIP_TRACE && !traces))
         return phip_align(b, out, traces);
     const int m = b->qlen[0], n = b->tlen[0];
+    if (b->dprof && b->dprof->m != m) {
+        phip_set_error("phip_align_one: the profile's device query has %d residues, not %d",
+                       b->dprof->m, m);
+        return -1;
+    }
     if (m <= 0 || n <= 0 ||
         !dp32_eligible_one(b->matrix, b->alg, b->kind, b->want_sat_check, b->open, b->gap, m, n))
         return phip_align(b, out, traces);
@@ -788,6 +805,42 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
     }
     pthread_mutex_unlock(&g_oq_lock);
     return me.rc;
+}
+
+extern "C" phip_dprofile_t *phip_profile_upload(const uint8_t *q, int m, const phip_matrix_t *mx)
+{
+    if (phip_init() != 0) return nullptr;
+    if (!q || m <= 0 || !mx || !mx->values || mx->size <= 0 || mx->length <= 0) {
+        phip_set_error("phip_profile_upload: bad arguments");
+        return nullptr;
+    }
+    const size_t qb = al256((size_t)m), mb = (size_t)mx->length * mx->size * 4;
+    phip_dprofile_t *p = (phip_dprofile_t *)calloc(1, sizeof(phip_dprofile_t));
+    char *d = nullptr;
+    if (!p || hipMalloc((void **)&d, qb + mb) != hipSuccess) {
+        phip_set_error("phip_profile_upload: out of memory (%zu bytes)", qb + mb);
+        free(p);
+        return nullptr;
+    }
+    if (hipMemcpy(d, q, (size_t)m, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + qb, mx->values, mb, hipMemcpyHostToDevice) != hipSuccess) {
+        phip_set_error("phip_profile_upload: upload failed");
+        (void)hipFree(d);
+        free(p);
+        return nullptr;
+    }
+    p->q = (uint8_t *)d;
+    p->mat = (int *)(d + qb);
+    p->m = m;
+    return p;
+}
+
+extern "C" void phip_profile_release(phip_dprofile_t *p)
+{
+    if (!p) return;
+    // hipFree waits for the device, so no launch still reads the copy
+    (void)hipFree(p->q);
+    free(p);
 }
 
 extern "C" int phip_align_tables(const phip_batch_t *b, int tab, phip_out_t *out, int **dst)

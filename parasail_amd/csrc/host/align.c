@@ -128,9 +128,12 @@ static int sg_result_flags(int sg_flags)
 
 /* tab: 0, or 1 for the _table and 2 for the _rowcol form of the function
  * (src/memory.c:171-275 results, src/sw.c:159-170 values) */
+/* qcodes/dprof: a profile's query codes (host) and their device copy with
+ * the matrix, or NULL to encode s1 here */
 static parasail_result_t *align_ex(const char *fname, int alg, int sg_flags, int kind, int width,
                                    int scan, int tab, const char *s1, int s1Len, const char *s2,
-                                   int s2Len, int open, int gap, const parasail_matrix_t *matrix)
+                                   int s2Len, int open, int gap, const parasail_matrix_t *matrix,
+                                   const uint8_t *qcodes, const phip_dprofile_t *dprof)
 {
     CHECK_NULL(fname, s2);
     CHECK_GT0(fname, s2Len);
@@ -163,7 +166,9 @@ static parasail_result_t *align_ex(const char *fname, int alg, int sg_flags, int
     uint8_t *q = NULL, *t = (uint8_t *)malloc((size_t)s2Len);
     if (s1) q = (uint8_t *)malloc((size_t)m);
     if (!t || (s1 && !q)) { free(q); free(t); return NULL; }
-    for (int i = 0; s1 && i < m; ++i) q[i] = (uint8_t)matrix->mapper[(unsigned char)s1[i]];
+    if (s1 && qcodes) memcpy(q, qcodes, (size_t)m);
+    for (int i = 0; s1 && !qcodes && i < m; ++i)
+        q[i] = (uint8_t)matrix->mapper[(unsigned char)s1[i]];
     for (int j = 0; j < s2Len; ++j) t[j] = (uint8_t)matrix->mapper[(unsigned char)s2[j]];
 
     phip_batch_t b;
@@ -191,6 +196,7 @@ static parasail_result_t *align_ex(const char *fname, int alg, int sg_flags, int
     b.t = t;
     b.toff = &to;
     b.tlen = &tl;
+    b.dprof = s1 ? dprof : NULL;
     phip_out_t o;
     int8_t *tr = NULL;
     parasail_result_t *rt = NULL; /* the table result, made before the call */
@@ -324,7 +330,27 @@ parasail_result_t *phost_align(const char *fname, int alg, int sg_flags, int kin
 {
     (void)profile;
     return align_ex(fname, alg, sg_flags, kind, width, scan, 0, s1, s1Len, s2, s2Len, open, gap,
-                    matrix);
+                    matrix, NULL, NULL);
+}
+
+/* The hip profile's data (the `score` member of the width's
+ * parasail_profile_data, freed by the profile's `free`): the query's
+ * residue codes and, for a square matrix, their device copy with the
+ * matrix values, uploaded when the profile is made. */
+typedef struct hip_pdata {
+    uint8_t *codes;
+    phip_dprofile_t *dev;
+} hip_pdata_t;
+
+/* the data for a call at width w (0: _sat), else any the profile holds */
+static const hip_pdata_t *profile_data(const parasail_profile_t *p, int w)
+{
+    const void *pick = w == 8 ? p->profile8.score : w == 32 ? p->profile32.score
+                                                            : p->profile16.score;
+    if (!pick) pick = p->profile16.score ? p->profile16.score
+                    : p->profile8.score  ? p->profile8.score
+                                         : p->profile32.score;
+    return (const hip_pdata_t *)pick;
 }
 
 static parasail_result_t *phost_align_profile(const char *fname, int alg, int sg_flags, int kind,
@@ -335,8 +361,9 @@ static parasail_result_t *phost_align_profile(const char *fname, int alg, int sg
     CHECK_NULL(fname, profile);
     CHECK_NULL(fname, profile->matrix);
     CHECK_GT0(fname, profile->s1Len);
+    const hip_pdata_t *d = profile_data(profile, width);
     return align_ex(fname, alg, sg_flags, kind, width, scan, tab, profile->s1, profile->s1Len, s2,
-                    s2Len, open, gap, profile->matrix);
+                    s2Len, open, gap, profile->matrix, d ? d->codes : NULL, d ? d->dev : NULL);
 }
 
 /* ---- the generated entry points ---- */
@@ -413,7 +440,7 @@ PARASAIL_HIP_ALGS(DEF_ALG)
         const int open, const int gap, const parasail_matrix_t *matrix)                      \
     {                                                                                        \
         return align_ex(__func__, ALG_##alg, SGF_##alg, KIND_##kind, WID_##w, SCAN_##strat,  \
-                        TAB_##tab, s1, s1Len, s2, s2Len, open, gap, matrix);                 \
+                        TAB_##tab, s1, s1Len, s2, s2Len, open, gap, matrix, NULL, NULL);     \
     }                                                                                        \
     PHOST_EXPORT parasail_result_t                                                           \
         *parasail_##alg##kind##_##tab##_##strat##_profile_hip_64_##w(                        \
@@ -436,11 +463,20 @@ PARASAIL_HIP_ALGS(DEF_TALG)
 
 /* ---- profiles ---- */
 
-static void profile_data_free(void *p) { free(p); }
+static void profile_data_free(void *p)
+{
+    hip_pdata_t *d = (hip_pdata_t *)p;
+    phip_profile_release(d->dev);
+    free(d->codes);
+    free(d);
+}
 
-/* The hip profile keeps the query's residue codes; the substitution table
- * travels with every launch (src/memory_avx2.c:81-132 builds striped score
- * vectors instead, which the wavefront kernels do not need). */
+/* The hip profile keeps the query's residue codes and, for a square
+ * matrix, a device copy of the codes and the matrix values that every
+ * per-call alignment with the profile reads in place of an upload
+ * (src/memory_avx2.c:81-132 builds striped score vectors instead, which
+ * the wavefront kernels do not need; src/memory.c:698-746 frees them
+ * through the profile's `free`). */
 static parasail_profile_t *profile_create(const char *fname, const char *s1, int s1Len,
                                           const parasail_matrix_t *matrix, int stats, int w)
 {
@@ -461,16 +497,28 @@ static parasail_profile_t *profile_create(const char *fname, const char *s1, int
     if (!p) return NULL;
     const int m = p->s1Len;
     uint8_t *codes = (uint8_t *)malloc((size_t)m + 1);
-    if (!codes) { free(p); return NULL; }
+    hip_pdata_t *pd = (hip_pdata_t *)calloc(1, sizeof(hip_pdata_t));
+    if (!codes || !pd) { free(codes); free(pd); free(p); return NULL; }
     for (int i = 0; i < m; ++i) codes[i] = s1 ? (uint8_t)matrix->mapper[(unsigned char)s1[i]] : 0;
-    p->free = profile_data_free;
-    struct parasail_profile_data *d = w == 8 ? &p->profile8 : w == 32 ? &p->profile32 : &p->profile16;
-    d->score = codes;
-    if (w == 0) {
-        /* _sat carries all widths (src/memory_avx2.c:240-253) */
-        p->profile8.score = NULL;
-        p->profile16.score = codes;
+    pd->codes = codes;
+    if (s1 && matrix->type == PARASAIL_MATRIX_TYPE_SQUARE) {
+        phip_matrix_t mx;
+        memset(&mx, 0, sizeof(mx));
+        mx.values = matrix->matrix;
+        mx.size = matrix->size;
+        mx.length = matrix->size;
+        pd->dev = phip_profile_upload(codes, m, &mx);
+        if (!pd->dev) {
+            fprintf(stderr, "%s: %s\n", fname, phip_last_error());
+            free(codes); free(pd); free(p);
+            return NULL;
+        }
     }
+    p->free = profile_data_free;
+    /* one width's slot; _sat's profile16 (src/memory_avx2.c:240-253 fills
+     * both 8 and 16 there, which share one copy here) */
+    struct parasail_profile_data *d = w == 8 ? &p->profile8 : w == 32 ? &p->profile32 : &p->profile16;
+    d->score = pd;
     return p;
 }
 

@@ -58,6 +58,10 @@ typedef struct phip_batch {
     const uint8_t *t;
     const int64_t *toff;
     const int32_t *tlen;
+    /* NULL, or the device copy of pair 0's query codes and of the matrix
+     * (phip_profile_upload) that phip_align_one uses instead of uploading
+     * q and matrix.values again */
+    const struct phip_dprofile *dprof;
 } phip_batch_t;
 
 typedef struct phip_out {
@@ -86,6 +90,14 @@ int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **traces);
 /* One pair (npairs == 1) with the lowest latency: one upload, one launch,
  * one download when the pair fits the 64-lane kernel, else phip_align. */
 int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **traces);
+
+/* A profile's query kept in device memory: its m residue codes and the
+ * matrix values (length x size ints, square or PSSM) uploaded once, for
+ * every per-call alignment made with the profile.  NULL on failure (the
+ * error in phip_last_error).  Released by phip_profile_release. */
+typedef struct phip_dprofile phip_dprofile_t;
+phip_dprofile_t *phip_profile_upload(const uint8_t *q, int m, const phip_matrix_t *mx);
+void phip_profile_release(phip_dprofile_t *p);
 
 /* One pair (npairs == 1, kind score or stats) with its DP tables: tab 1
  * writes every cell's H (and, for stats, the matches, similar and length of

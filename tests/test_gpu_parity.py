@@ -241,6 +241,70 @@ def test_profile_functions(mats):
         L.parasail_profile_free(prof)
 
 
+def test_profile_device_copy(mats):
+    """A hip profile over a square matrix holds a device copy of its query
+    codes and matrix, made once by the creator and freed by the profile's
+    `free` (src/memory.c:698-746).  Every profile call reads that copy: a
+    score call after the caller's s1 buffer is overwritten still aligns the
+    query the profile was made from, as the reference's striped profile,
+    built at creation (src/memory_avx2.c:81-132), does.  Score and stats
+    calls of every width equal the oracle."""
+    L = P.lib()
+    pm, om = mats["blosum62"]
+    rng = random.Random(12)
+    q = rseq(rng, 240)
+
+    class PData(P.ctypes.Structure):
+        _fields_ = [("score", P.ctypes.c_void_p), ("matches", P.ctypes.c_void_p),
+                    ("similar", P.ctypes.c_void_p)]
+
+    class Profile(P.ctypes.Structure):
+        _fields_ = [("s1", P.ctypes.c_void_p), ("s1Len", P.ctypes.c_int),
+                    ("matrix", P.ctypes.c_void_p), ("p8", PData), ("p16", PData),
+                    ("p32", PData), ("p64", PData), ("free", P.ctypes.c_void_p),
+                    ("stop", P.ctypes.c_int)]
+
+    class HipData(P.ctypes.Structure):
+        _fields_ = [("codes", P.ctypes.c_void_p), ("dev", P.ctypes.c_void_p)]
+
+    L.parasail_profile_free.argtypes = [P.ctypes.c_void_p]
+    targets = [rseq(rng, rng.randint(30, 700)) for _ in range(6)]
+    for kind, w, slot in (("", "16", "p16"), ("", "8", "p8"), ("", "32", "p32"),
+                          ("", "sat", "p16"), ("_stats", "16", "p16"), ("_stats", "sat", "p16")):
+        fn = "parasail_sw%s_striped_profile_hip_64_%s" % (kind, w)
+        create = P.ctypes.CFUNCTYPE(P.ctypes.c_void_p, P.ctypes.c_void_p, P.ctypes.c_int, P.MP)(
+            L.parasail_lookup_pcreator(fn.encode()))
+        pf = P.ctypes.CFUNCTYPE(P.RP, P.ctypes.c_void_p, P.ctypes.c_char_p, P.ctypes.c_int,
+                                P.ctypes.c_int, P.ctypes.c_int)(
+            L.parasail_lookup_pfunction(fn.encode()))
+        buf = P.ctypes.create_string_buffer(q.encode())
+        prof = create(P.ctypes.addressof(buf), len(q), pm)
+        assert prof, fn
+        pr = Profile.from_address(prof)
+        data = getattr(pr, slot)
+        assert data.score and not data.matches and not data.similar, fn
+        hd = HipData.from_address(data.score)
+        assert hd.codes and hd.dev, fn
+        if not kind:
+            # the profile no longer reads the caller's residues
+            P.ctypes.memmove(buf, ("W" * len(q)).encode(), len(q))
+        for t in targets:
+            r = pf(prof, t.encode(), len(t), 11, 1)
+            assert r, fn
+            alg_kind = O.STATS if kind else O.SCORE
+            exp, _ = O.align(O.SW, alg_kind, q, t, 11, 1, om)
+            rc = r.contents
+            if rc.flag & P.FLAG_SATURATED:
+                assert w == "8" and exp[0] > (1 << 8) - 11 - 2, (fn, t)
+            else:
+                assert (rc.score, rc.end_query, rc.end_ref) == exp[:3], (fn, t)
+                if kind:
+                    st = P.ctypes.cast(rc.extra, P.ctypes.POINTER(P.StatsExtra)).contents
+                    assert (st.matches, st.similar, st.length) == exp[3:6], (fn, t)
+            L.parasail_result_free(r)
+        L.parasail_profile_free(prof)
+
+
 def test_config0_all_targets():
     """The per-pair workload in full: the 300-residue query against all
     2,000 targets through parasail_sw_striped_profile_hip_64_16 (and its
