@@ -147,36 +147,51 @@ def per_call_gcups(reps=1, threads=1):
     return b.cells / best / 1e9, b.cells, best
 
 
-def cpu_baseline(batch, target_s=15.0):
-    """The CPU oracle (serial recurrence per pair, OpenMP over pairs:
-    tests/test_openmp.c:383-441) on a bounded sample of the batch."""
+def cpu_baseline(batch, gpu=None, target_s=15.0):
+    """The reference's dispatched CPU path restated: AVX2 Farrar striped
+    query profile, one profile per pair, 8-bit lanes first, then 16-bit,
+    then the serial recurrence (the `_striped_sat` chain), OpenMP over pairs
+    (tests/test_openmp.c:353-369), on a bounded sample of the batch.  gpu:
+    the GPU's results of the batch, {alg: {"score", "end_query", "end_ref"}},
+    compared with the CPU's on the sample."""
     import oracle as O
     om = O.blosum62()
     # the host's share of cores per GPU is 16, whatever the OS reports
     threads = max(1, min(os.cpu_count() or 1, 16))
     rng = np.random.default_rng(9)
-    idx = rng.choice(len(batch), size=min(len(batch), 200), replace=False)
     algs = {"sw": (O.SW, (0, 0, 0, 0)), "nw": (O.NW, (0, 0, 0, 0)), "sg": (O.SG, (1, 1, 1, 1))}
+    if not O.striped_available():
+        return None
 
     def run(ids):
         qs = [om.map(batch.query(k)) for k in ids]
         ts = [om.map(batch.target(k)) for k in ids]
         cells = sum(len(a) * len(b) for a, b in zip(qs, ts))
-        secs = 0.0
-        for alg, flags in algs.values():
-            _, s = O.batch(alg, O.SCORE, qs, ts, 11, 1, om, flags, threads=threads)
+        secs, same, widths = 0.0, True, {}
+        for name, (alg, flags) in algs.items():
+            res, w, s = O.striped_batch(alg, qs, ts, 11, 1, om, flags, threads=threads)
+            if s < 0:
+                raise RuntimeError("striped baseline failed")
             secs += s
-        return 3 * cells, secs
+            widths[name] = {str(b): int(np.count_nonzero(w == b)) for b in (8, 16, 32)}
+            if gpu is not None and name in gpu:
+                g = np.stack([gpu[name][k][ids] for k in ("score", "end_query", "end_ref")], 1)
+                same &= bool((g == res).all())
+        return 3 * cells, secs, same, widths
 
-    cells, secs = run(idx)
+    idx = rng.choice(len(batch), size=min(len(batch), 400), replace=False)
+    _, secs, _, _ = run(idx)
     scale = max(1, int(target_s / max(secs, 1e-3)))
     n2 = min(len(batch), len(idx) * scale)
     idx = rng.choice(len(batch), size=n2, replace=False)
-    cells, secs = run(idx)
+    cells, secs, same, widths = run(idx)
     return {"value": round(cells / secs / 1e9, 3), "unit": "GCUPS", "cores": threads,
             "kind": "port",
-            "sample": "%d random pairs of the batch x (SW, NW, SG), serial int32 oracle, "
-                      "OpenMP over pairs, %.1f s" % (n2, secs)}
+            "sample": "%d random pairs of the batch x (SW, NW, SG), AVX2 striped query profile "
+                      "per pair, int8 then int16 then int32 (the reference's dispatched "
+                      "_striped_sat chain restated), OpenMP over pairs, %.1f s" % (n2, secs),
+            "widths_held": widths,
+            "equals_gpu_on_sample": same if gpu is not None else None}
 
 
 def serve_load(db, pm, nres, nreq, qps):
@@ -608,7 +623,7 @@ def main():
         line["serving"] = serving(a.serve_requests, a.qps,
                                   [float(x) for x in a.qps_sweep.split(",") if x])
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(batch)
+        line["cpu_baseline"] = cpu_baseline(batch, gpu=bufs)
     print(json.dumps(line), flush=True)
 
 

@@ -88,6 +88,15 @@ def lib():
             P(ctypes.c_int), P(ctypes.c_int64), P(ctypes.c_int),
             P(ctypes.c_int), P(ctypes.c_int64), P(ctypes.c_int),
             ctypes.c_int, ctypes.c_int, P(OrcMatrix), P(OrcResult), ctypes.c_int]
+        _lib.orc_striped_sat.argtypes = [ctypes.c_int] * 5 + [
+            P(ctypes.c_int), ctypes.c_int, P(ctypes.c_int), ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, P(OrcMatrix), P(OrcResult)]
+        _lib.orc_striped_batch.restype = ctypes.c_double
+        _lib.orc_striped_batch.argtypes = [ctypes.c_int] * 6 + [
+            P(ctypes.c_int), P(ctypes.c_int64), P(ctypes.c_int),
+            P(ctypes.c_int), P(ctypes.c_int64), P(ctypes.c_int),
+            ctypes.c_int, ctypes.c_int, P(OrcMatrix), P(OrcResult), P(ctypes.c_int),
+            ctypes.c_int]
     return _lib
 
 
@@ -282,3 +291,47 @@ def batch(alg, kind, queries, targets, open_, gap, matrix, sg_flags=(0, 0, 0, 0)
                            _iptr(tlen), open_, gap, matrix.ptr, out, threads)
     res = np.array([out[k].as_tuple() for k in range(n)], dtype=np.int64).reshape(n, 6)
     return res, secs
+
+
+def striped_available():
+    """Whether the host has AVX2 for the striped baseline."""
+    return bool(lib().orc_striped_available())
+
+
+def striped_sat(alg, s1, s2, open_, gap, matrix, sg_flags=(0, 0, 0, 0)):
+    """One pair through the AVX2 striped 8 -> 16 -> 32-bit chain (score
+    only).  Returns ((score, end_query, end_ref), width that held)."""
+    a = np.ascontiguousarray(matrix.map(s1) if isinstance(s1, (str, bytes)) else s1, dtype=np.int32)
+    b = np.ascontiguousarray(matrix.map(s2) if isinstance(s2, (str, bytes)) else s2, dtype=np.int32)
+    res = OrcResult()
+    w = lib().orc_striped_sat(alg, *sg_flags, _iptr(a), len(a), _iptr(b), len(b), open_, gap,
+                              matrix.ptr, ctypes.byref(res))
+    if w < 0:
+        return None, w
+    return (res.score, res.end_query, res.end_ref), w
+
+
+def striped_batch(alg, queries, targets, open_, gap, matrix, sg_flags=(0, 0, 0, 0), threads=0):
+    """The AVX2 striped chain over every pair, OpenMP over pairs.  queries
+    and targets are lists of mapped int32 arrays.  Returns (results[n,3],
+    widths[n], seconds)."""
+    n = len(queries)
+    qlen = np.array([len(x) for x in queries], dtype=np.int32)
+    tlen = np.array([len(x) for x in targets], dtype=np.int32)
+    qoff = np.zeros(n, dtype=np.int64)
+    toff = np.zeros(n, dtype=np.int64)
+    if n:
+        qoff[1:] = np.cumsum(qlen)[:-1]
+        toff[1:] = np.cumsum(tlen)[:-1]
+    q = np.ascontiguousarray(np.concatenate(queries) if n else np.zeros(1), dtype=np.int32)
+    t = np.ascontiguousarray(np.concatenate(targets) if n else np.zeros(1), dtype=np.int32)
+    out = (OrcResult * max(n, 1))()
+    width = np.zeros(max(n, 1), dtype=np.int32)
+    secs = lib().orc_striped_batch(alg, *sg_flags, n, _iptr(q),
+                                   qoff.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                   _iptr(qlen), _iptr(t),
+                                   toff.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                   _iptr(tlen), open_, gap, matrix.ptr, out, _iptr(width),
+                                   threads)
+    res = np.array([out[k].as_tuple()[:3] for k in range(n)], dtype=np.int64).reshape(n, 3)
+    return res, width[:n], secs

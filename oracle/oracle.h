@@ -121,6 +121,21 @@ double orc_batch(int alg, int kind, int s1_beg, int s1_end, int s2_beg,
                  const int *tlen, int open, int gap, const orc_matrix_t *mx,
                  orc_result_t *out, int threads);
 
+/* ---- AVX2 striped baseline (striped.c) ----
+ * Score-only SW/NW/SG through the reference's dispatched `_striped_sat`
+ * chain restated: Farrar striped query profile in AVX2, 8-bit lanes, then
+ * 16-bit, then the serial recurrence.  Square matrices only.
+ * orc_striped_sat returns the width that held (8, 16, 32) or -1 (no AVX2,
+ * bad arguments); orc_striped_batch the wall seconds (-1 on failure). */
+int orc_striped_available(void);
+int orc_striped_sat(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, const int *q,
+                    int m, const int *t, int n, int open, int gap, const orc_matrix_t *mx,
+                    orc_result_t *out);
+double orc_striped_batch(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, int npairs,
+                         const int *q, const int64_t *qoff, const int *qlen, const int *t,
+                         const int64_t *toff, const int *tlen, int open, int gap,
+                         const orc_matrix_t *mx, orc_result_t *out, int *width, int threads);
+
 #ifdef __cplusplus
 }
 #endif

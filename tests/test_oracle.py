@@ -702,3 +702,47 @@ def test_stats_tables_are_counts_along_trace_paths():
                     exp = counts_along_cigar(c[0], c[1], c[2], i, j, q, t, mx.values())
                     got = (tabs[1][i, j], tabs[2][i, j], tabs[3][i, j])
                     assert got == exp, (alg, flags, s1, s2, o, g, i, j)
+
+
+def test_striped_baseline_equals_serial():
+    """The CPU baseline bench.py times (the AVX2 striped 8 -> 16 -> 32-bit
+    chain, oracle/striped.c) returns the serial recurrence's score and
+    ends for SW, NW and SG under all 16 end-flag sets, at every width tier:
+    query lengths around the 32- and 16-lane segment boundaries, the
+    reference's gap sets (open > gap, tests/test_verify.c:130-190) and the
+    benchmarks' 11/1 and 5/2, pairs that hold at 8 bits, need 16, and
+    (a W-rich self pair past 32,767) need 32."""
+    if not O.striped_available():
+        pytest.skip("host without AVX2")
+    rng = random.Random(77)
+    b = O.blosum62()
+    dna = O.matrix_create("ACGT", 2, -3)
+    aa = "ARNDCQEGHILKMFPSTWYV"
+    seen = set()
+    for trial in range(900):
+        mx, alpha = (dna, "ACGT") if trial % 5 == 0 else (b, aa)
+        m = rng.choice([1, 2, 15, 16, 17, 31, 32, 33, 63, 65, 200, 700])
+        n = rng.choice([1, 3, 40, 300, 900])
+        s1 = rand_seq(rng, alpha, m)
+        s2 = rand_seq(rng, alpha, n)
+        if rng.random() < 0.3:
+            s2 = s1[:n] + s2[: max(0, n - m)]
+        alg = rng.choice([O.SW, O.NW, O.SG])
+        flags = rng.choice(SG_FLAGS) if alg == O.SG else (0, 0, 0, 0)
+        o, g = rng.choice([(10, 1), (10, 2), (14, 2), (40, 2), (11, 1), (5, 2)])
+        exp, _ = O.align(alg, O.SCORE, s1, s2, o, g, mx, flags)
+        got, w = O.striped_sat(alg, s1, s2, o, g, mx, flags)
+        assert got == exp[:3], (alg, flags, m, n, o, g, w)
+        seen.add((alg, w))
+    # SW's 16-bit lanes are biased by -32768 and hold scores up to
+    # 2^16 - max - 2 = 65,523 (sw_striped_avx2_256_16.c:205-210,350-362);
+    # NW and SG's signed ones up to 32,767 - max - 1
+    for alg, length in ((O.SW, 3100), (O.SW, 6000), (O.NW, 3100), (O.SG, 3100)):
+        s = "W" * length
+        exp, _ = O.align(alg, O.SCORE, s, s, 11, 1, b, (1, 1, 1, 1))
+        got, w = O.striped_sat(alg, s, s, 11, 1, b, (1, 1, 1, 1))
+        assert exp[0] == 11 * length
+        assert (got, w) == (exp[:3], 16 if (alg, length) == (O.SW, 3100) else 32)
+        seen.add((alg, w))
+    for alg in (O.SW, O.NW, O.SG):
+        assert {(alg, 8), (alg, 16), (alg, 32)} <= seen, (alg, seen)
