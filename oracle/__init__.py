@@ -96,7 +96,7 @@ def lib():
             P(ctypes.c_int), P(ctypes.c_int64), P(ctypes.c_int),
             P(ctypes.c_int), P(ctypes.c_int64), P(ctypes.c_int),
             ctypes.c_int, ctypes.c_int, P(OrcMatrix), P(OrcResult), P(ctypes.c_int),
-            ctypes.c_int]
+            ctypes.c_int, ctypes.c_int]
     return _lib
 
 
@@ -311,10 +311,12 @@ def striped_sat(alg, s1, s2, open_, gap, matrix, sg_flags=(0, 0, 0, 0)):
     return (res.score, res.end_query, res.end_ref), w
 
 
-def striped_batch(alg, queries, targets, open_, gap, matrix, sg_flags=(0, 0, 0, 0), threads=0):
-    """The AVX2 striped chain over every pair, OpenMP over pairs.  queries
-    and targets are lists of mapped int32 arrays.  Returns (results[n,3],
-    widths[n], seconds)."""
+def striped_batch(alg, queries, targets, open_, gap, matrix, sg_flags=(0, 0, 0, 0), threads=0,
+                  start=8):
+    """The AVX2 striped chain from `start` bits (8, or 16 as the _16
+    functions) over every pair, OpenMP over pairs.  queries and targets are
+    lists of mapped int32 arrays.  Returns (results[n,3], widths[n],
+    seconds)."""
     n = len(queries)
     qlen = np.array([len(x) for x in queries], dtype=np.int32)
     tlen = np.array([len(x) for x in targets], dtype=np.int32)
@@ -332,6 +334,6 @@ def striped_batch(alg, queries, targets, open_, gap, matrix, sg_flags=(0, 0, 0, 
                                    _iptr(qlen), _iptr(t),
                                    toff.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                    _iptr(tlen), open_, gap, matrix.ptr, out, _iptr(width),
-                                   threads)
+                                   threads, start)
     res = np.array([out[k].as_tuple()[:3] for k in range(n)], dtype=np.int64).reshape(n, 3)
     return res, width[:n], secs

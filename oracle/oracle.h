@@ -125,16 +125,22 @@ double orc_batch(int alg, int kind, int s1_beg, int s1_end, int s2_beg,
  * Score-only SW/NW/SG through the reference's dispatched `_striped_sat`
  * chain restated: Farrar striped query profile in AVX2, 8-bit lanes, then
  * 16-bit, then the serial recurrence.  Square matrices only.
- * orc_striped_sat returns the width that held (8, 16, 32) or -1 (no AVX2,
- * bad arguments); orc_striped_batch the wall seconds (-1 on failure). */
+ * orc_striped_chain starts at `start` bits (8, or 16 as the _16 functions
+ * do) and returns the width that held (8, 16, 32) or -1 (no AVX2, bad
+ * arguments); orc_striped_sat is the chain from 8 bits; orc_striped_batch
+ * returns the wall seconds (-1 on failure). */
 int orc_striped_available(void);
+int orc_striped_chain(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, const int *q,
+                      int m, const int *t, int n, int open, int gap, const orc_matrix_t *mx,
+                      orc_result_t *out, int start);
 int orc_striped_sat(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, const int *q,
                     int m, const int *t, int n, int open, int gap, const orc_matrix_t *mx,
                     orc_result_t *out);
 double orc_striped_batch(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, int npairs,
                          const int *q, const int64_t *qoff, const int *qlen, const int *t,
                          const int64_t *toff, const int *tlen, int open, int gap,
-                         const orc_matrix_t *mx, orc_result_t *out, int *width, int threads);
+                         const orc_matrix_t *mx, orc_result_t *out, int *width, int threads,
+                         int start);
 
 #ifdef __cplusplus
 }

@@ -30,17 +30,21 @@
 
 int orc_striped_available(void) { return __builtin_cpu_supports("avx2") ? 1 : 0; }
 
-/* One pair through the chain; returns the width that held (8, 16, 32) or
- * -1.  s1_beg.. are the SG end flags (ignored for SW and NW). */
-int orc_striped_sat(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, const int *q,
-                    int m, const int *t, int n, int open, int gap, const orc_matrix_t *mx,
-                    orc_result_t *out)
+/* One pair through the chain from `start` bits (8, or 16 for the _16
+ * functions); returns the width that held (8, 16, 32) or -1.  s1_beg.. are
+ * the SG end flags (ignored for SW and NW). */
+int orc_striped_chain(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, const int *q,
+                      int m, const int *t, int n, int open, int gap, const orc_matrix_t *mx,
+                      orc_result_t *out, int start)
 {
     if (!orc_striped_available() || m <= 0 || n <= 0 || mx->type != 0) return -1;
     memset(out, 0, sizeof(*out));
-    int rc = st_align8(alg, s1_beg, s1_end, s2_beg, s2_end, q, m, t, n, open, gap, mx, out);
-    if (rc == 0) return 8;
-    if (rc < 0) return -1;
+    int rc;
+    if (start <= 8) {
+        rc = st_align8(alg, s1_beg, s1_end, s2_beg, s2_end, q, m, t, n, open, gap, mx, out);
+        if (rc == 0) return 8;
+        if (rc < 0) return -1;
+    }
     rc = st_align16(alg, s1_beg, s1_end, s2_beg, s2_end, q, m, t, n, open, gap, mx, out);
     if (rc == 0) return 16;
     if (rc < 0) return -1;
@@ -50,6 +54,14 @@ int orc_striped_sat(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, con
     return 32;
 }
 
+int orc_striped_sat(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, const int *q,
+                    int m, const int *t, int n, int open, int gap, const orc_matrix_t *mx,
+                    orc_result_t *out)
+{
+    return orc_striped_chain(alg, s1_beg, s1_end, s2_beg, s2_end, q, m, t, n, open, gap, mx, out,
+                             8);
+}
+
 static double now_s(void)
 {
     struct timespec ts;
@@ -57,14 +69,15 @@ static double now_s(void)
     return ts.tv_sec + ts.tv_nsec * 1e-9;
 }
 
-/* Every pair through orc_striped_sat, OpenMP over pairs with guided
+/* Every pair through orc_striped_chain from `start` bits, OpenMP over pairs with guided
  * scheduling (tests/test_openmp.c:353-369); width[p] (may be NULL) gets
  * the width that held.  Returns the wall seconds, or -1 when a pair
  * failed. */
 double orc_striped_batch(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end, int npairs,
                          const int *q, const int64_t *qoff, const int *qlen, const int *t,
                          const int64_t *toff, const int *tlen, int open, int gap,
-                         const orc_matrix_t *mx, orc_result_t *out, int *width, int threads)
+                         const orc_matrix_t *mx, orc_result_t *out, int *width, int threads,
+                         int start)
 {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
@@ -77,8 +90,9 @@ double orc_striped_batch(int alg, int s1_beg, int s1_end, int s2_beg, int s2_end
 #pragma omp parallel for schedule(guided) reduction(| : failed)
 #endif
     for (int p = 0; p < npairs; ++p) {
-        const int w = orc_striped_sat(alg, s1_beg, s1_end, s2_beg, s2_end, q + qoff[p], qlen[p],
-                                      t + toff[p], tlen[p], open, gap, mx, &out[p]);
+        const int w = orc_striped_chain(alg, s1_beg, s1_end, s2_beg, s2_end, q + qoff[p],
+                                        qlen[p], t + toff[p], tlen[p], open, gap, mx, &out[p],
+                                        start);
         if (width) width[p] = w;
         failed |= w < 0;
     }

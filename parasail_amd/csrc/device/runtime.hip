@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <algorithm>
 #include <vector>
@@ -613,7 +614,10 @@ int one_reserve(size_t bytes)
 // that shares its scoring (algorithm, kind, end flags, gaps, matrix and
 // saturation check; at most kOneMax) and runs them as one batch of 64-lane
 // jobs with one upload, one launch and one download; the others wait for
-// their results.  A lone caller is a batch of one.  Pairs outside the
+// their results.  A lone caller is a batch of one.  Callers that were in
+// one launch come back together, so a leader first waits, at most
+// kGatherUs, until as many calls are queued as its predecessor launched
+// (the first caller back would otherwise launch alone).  Pairs outside the
 // kernel's range take phip_align.
 namespace {
 
@@ -627,9 +631,12 @@ struct OneReq {
 
 pthread_mutex_t g_oq_lock = PTHREAD_MUTEX_INITIALIZER;
 pthread_cond_t g_oq_cv = PTHREAD_COND_INITIALIZER;
+pthread_cond_t g_oq_gather_cv = PTHREAD_COND_INITIALIZER;
 std::vector<OneReq *> g_oq;
-bool g_oq_busy = false;
+bool g_oq_busy = false, g_oq_gathering = false;
+int g_oq_last = 1;  // calls in the previous launch
 constexpr int kOneMax = 512;
+constexpr long kGatherUs = 200;
 
 bool same_scoring(const phip_batch_t *a, const phip_batch_t *b)
 {
@@ -781,14 +788,30 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
     OneReq me{b, out, traces, -1, false};
     pthread_mutex_lock(&g_oq_lock);
     g_oq.push_back(&me);
+    if (g_oq_gathering) pthread_cond_signal(&g_oq_gather_cv);
     while (!me.done) {
         if (g_oq_busy) {
             pthread_cond_wait(&g_oq_cv, &g_oq_lock);
             continue;
         }
+        g_oq_busy = true;
+        // gather: the previous launch's callers are on their way back
+        if ((int)g_oq.size() < g_oq_last) {
+            struct timespec dl;
+            clock_gettime(CLOCK_REALTIME, &dl);
+            dl.tv_nsec += kGatherUs * 1000;
+            if (dl.tv_nsec >= 1000000000L) {
+                dl.tv_sec += 1;
+                dl.tv_nsec -= 1000000000L;
+            }
+            g_oq_gathering = true;
+            while ((int)g_oq.size() < g_oq_last &&
+                   pthread_cond_timedwait(&g_oq_gather_cv, &g_oq_lock, &dl) == 0) {
+            }
+            g_oq_gathering = false;
+        }
         // lead: the oldest request's scoring, every queued request that
         // shares it
-        g_oq_busy = true;
         std::vector<OneReq *> batch, rest;
         const phip_batch_t *key = g_oq.front()->b;
         for (OneReq *r : g_oq) {
@@ -796,6 +819,7 @@ extern "C" int phip_align_one(const phip_batch_t *b, phip_out_t *out, int8_t **t
             else rest.push_back(r);
         }
         g_oq.swap(rest);
+        g_oq_last = (int)batch.size();
         pthread_mutex_unlock(&g_oq_lock);
         one_batch(batch);
         pthread_mutex_lock(&g_oq_lock);
