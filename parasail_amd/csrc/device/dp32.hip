@@ -657,33 +657,17 @@ bool dp32_plan(const HostBatch &hb, const std::vector<int> &pairs, const int32_t
         keys.push_back(k);
     }
     // jobs of one shape share waves; longest first so a wave's jobs end
-    // together and the longest waves start first: (T, R, n) descending, by
-    // a stable radix sort of 32-bit keys (a comparison sort of a large batch,
-    // most of whose keys tie, costs milliseconds)
+    // together and the longest waves start first: (T, R, n) descending
     {
         const size_t N = keys.size();
-        std::vector<uint32_t> k(N), k2(N);
-        std::vector<int> ix(N), ix2(N);
+        std::vector<uint32_t> k(N);
         for (size_t i = 0; i < N; ++i) {
             const int tr = keys[i].T == 64 ? 0 : keys[i].T == 32 ? 1 : 2;
             const uint32_t rank = (uint32_t)(tr * 64 + (63 - std::min(keys[i].R, 63)));
             const uint32_t nn = (uint32_t)std::min(keys[i].n, 0xFFFFFF);
             k[i] = (rank << 24) | (0xFFFFFFu - nn);
-            ix[i] = (int)i;
         }
-        std::vector<uint32_t> cnt(65537);
-        for (int shift = 0; shift < 32; shift += 16) {
-            std::fill(cnt.begin(), cnt.end(), 0u);
-            for (size_t i = 0; i < N; ++i) ++cnt[((k[i] >> shift) & 0xFFFF) + 1];
-            for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
-            for (size_t i = 0; i < N; ++i) {
-                const size_t pos = cnt[(k[i] >> shift) & 0xFFFF]++;
-                k2[pos] = k[i];
-                ix2[pos] = ix[i];
-            }
-            k.swap(k2);
-            ix.swap(ix2);
-        }
+        const std::vector<int> ix = radix_order(k);
         std::vector<Key> sorted(N);
         for (size_t i = 0; i < N; ++i) sorted[i] = keys[ix[i]];
         keys.swap(sorted);

@@ -19,6 +19,11 @@ struct HostBatch {
 void *dev_alloc(size_t bytes);
 void dev_free(void *p);
 
+// The order that sorts 32-bit keys ascending, ties in index order (a
+// stable radix sort in two 16-bit digits: a comparison sort of a large
+// batch, most of whose keys tie, costs milliseconds).
+std::vector<int> radix_order(const std::vector<uint32_t> &keys);
+
 struct WavePlan;
 
 // Plans and uploads everything that depends only on the batch and the

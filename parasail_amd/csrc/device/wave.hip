@@ -689,7 +689,9 @@ double job_cycles(bool sw, int T, int R, int nb, int n)
 }
 
 // Band height T*R and band count for a query of m rows against n columns:
-// the cheapest (rows computed x steps) cover of the query.
+// the cheapest (rows computed x steps) cover of the query.  The columns
+// scale every candidate's cost alike (n + T - 1 steps a band), so the
+// choice depends on m alone; choose_m memoises it per query length.
 Shape choose(bool sw, int T, int m, int n)
 {
     Shape best{kRmax, (m + T * kRmax - 1) / (T * kRmax), 0};
@@ -705,6 +707,22 @@ Shape choose(bool sw, int T, int m, int n)
     }
     return best;
 }
+
+// choose() for T = 16 through a table indexed by m (filled on first use);
+// the cost for n columns is nb * (n + T - 1) * step_cost(R)
+struct ShapeTable {
+    bool sw;
+    std::vector<Shape> by_m;
+    Shape get(int m, int n)
+    {
+        if ((size_t)m >= by_m.size()) by_m.resize((size_t)m + 1, Shape{0, 0, 0});
+        Shape &s = by_m[m];
+        if (s.nb == 0) s = choose(sw, 16, m, 1);
+        Shape r = s;
+        r.cost = job_cycles(sw, 16, r.R, r.nb, n);
+        return r;
+    }
+};
 
 typedef void (*KFn)(WaveCfg);
 
@@ -785,6 +803,28 @@ struct WavePlan {
     char *dlong = nullptr;
 };
 
+std::vector<int> radix_order(const std::vector<uint32_t> &keys)
+{
+    const size_t N = keys.size();
+    std::vector<uint32_t> k(keys), k2(N);
+    std::vector<int> ix(N), ix2(N);
+    for (size_t i = 0; i < N; ++i) ix[i] = (int)i;
+    std::vector<uint32_t> cnt(65537);
+    for (int shift = 0; shift < 32; shift += 16) {
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (size_t i = 0; i < N; ++i) ++cnt[((k[i] >> shift) & 0xFFFF) + 1];
+        for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+        for (size_t i = 0; i < N; ++i) {
+            const size_t pos = cnt[(k[i] >> shift) & 0xFFFF]++;
+            k2[pos] = k[i];
+            ix2[pos] = ix[i];
+        }
+        k.swap(k2);
+        ix.swap(ix2);
+    }
+    return ix;
+}
+
 bool wave_plan_matches(const WavePlan *p, int alg, int sg_flags, int open, int gap, int check)
 {
     return p->alg == alg && p->sg_flags == sg_flags && p->open == open && p->gap == gap &&
@@ -831,12 +871,13 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     const bool table_ok = (size_t)(mx.size + 2) * (tsp0 / 2) * 4 <= 65535u;
     double total = 0;
     std::vector<int> lng;
+    ShapeTable st{sw, {}};
     for (int i = 0; i < np; ++i) {
         P->maxn = std::max(P->maxn, (int)hb.tlen[i]);
         if (table_ok && int16_fits(mx, dp.alg, dp.open, dp.gap, hb.qlen[i], hb.tlen[i])) {
             fast.push_back(i);
             if (!sw && dp.check && !window_settled(mx, dp, hb.qlen[i], hb.tlen[i])) unsettled[i] = 1;
-            shape[i] = choose(sw, 16, hb.qlen[i], hb.tlen[i]);
+            shape[i] = st.get(hb.qlen[i], hb.tlen[i]);
             total += shape[i].cost;
         } else if (nwlong_eligible(mx, dp.alg, dp.kind, dp.check, hb.qlen[i], hb.tlen[i])) {
             lng.push_back(i);
@@ -855,13 +896,21 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
             pT[i] = 64;
         }
     }
-    // class (T, R), then longest first
-    std::sort(fast.begin(), fast.end(), [&](int x, int y) {
-        if (pT[x] != pT[y]) return pT[x] > pT[y];
-        if (shape[x].R != shape[y].R) return shape[x].R < shape[y].R;
-        if (shape[x].nb != shape[y].nb) return shape[x].nb > shape[y].nb;
-        return hb.tlen[x] > hb.tlen[y];
-    });
+    // class (T, R), then longest first: T 64 before 16, R ascending, bands
+    // and then columns descending (nb <= 512, R <= 20, n <= 30000 for a pair
+    // int16_fits admits)
+    {
+        std::vector<uint32_t> key(fast.size());
+        for (size_t x = 0; x < fast.size(); ++x) {
+            const int i = fast[x];
+            key[x] = (uint32_t)(pT[i] == 64 ? 0 : 1) << 30 | (uint32_t)shape[i].R << 25 |
+                     (uint32_t)(1023 - shape[i].nb) << 15 | (uint32_t)(32767 - hb.tlen[i]);
+        }
+        const std::vector<int> ord = radix_order(key);
+        std::vector<int> sorted(fast.size());
+        for (size_t x = 0; x < fast.size(); ++x) sorted[x] = fast[ord[x]];
+        fast.swap(sorted);
+    }
     // LDS table: rows [0,size) real, VIRT, PADQ; columns [0,size) real, PADT
     const int size = mx.size;
     const int TS = size + 1;
@@ -900,6 +949,11 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     std::vector<int> jT, jR;
     std::vector<char> jext;
     std::vector<double> jcost;
+    jobs.reserve(fast.size());
+    jT.reserve(fast.size());
+    jR.reserve(fast.size());
+    jext.reserve(fast.size());
+    jcost.reserve(fast.size());
     size_t k = 0;
     while (k < fast.size()) {
         WaveJob J;
@@ -935,6 +989,8 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     }
     std::vector<WaveDesc> waves;
     std::vector<double> wcost;
+    waves.reserve(jobs.size());
+    wcost.reserve(jobs.size());
     {
         size_t j0 = 0;
         while (j0 < jobs.size()) {
@@ -970,6 +1026,11 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
     std::vector<WaveJob> qj;
     std::vector<int> clen, cT;
     std::vector<PairRef> refs;
+    qw.reserve(waves.size());
+    qj.reserve(jobs.size());
+    clen.reserve(jobs.size());
+    cT.reserve(jobs.size());
+    refs.reserve(fast.size());
     int64_t ccoff = 0, bboff = 0, cdoff = 0;
     int maxw = 0;
     for (int wi : order) {

@@ -546,9 +546,11 @@ def test_batch_composition_independent(mats):
 
 def test_config1_full_size_properties(mats):
     """The full 200,000-pair protein batch (log-normal lengths around 300,
-    BLOSUM62 11/1): a random 2,000-pair sample equals the oracle bit for bit;
-    on every pair SW >= SG (all ends free) >= NW, SW >= 0, the end positions
-    lie inside the sequences, and a second pass reproduces the first."""
+    BLOSUM62 11/1): a random 2,000-pair sample equals the serial oracle bit
+    for bit, and every pair equals the oracle's AVX2 striped restatement
+    (test_oracle.py pins it to the serial oracle); on every pair SW >= SG
+    (all ends free) >= NW, SW >= 0, the end positions lie inside the
+    sequences, and a second pass reproduces the first."""
     from parasail_amd import workloads as WL
     pm, om = mats["blosum62"]
     b = WL.config1()
@@ -571,6 +573,19 @@ def test_config1_full_size_properties(mats):
         got = np.stack([r["score"][idx], r["end_query"][idx], r["end_ref"][idx]], axis=1)
         bad = np.nonzero((got != exp[:, :3]).any(axis=1))[0]
         assert bad.size == 0, (name, [(int(idx[k]), got[k].tolist(), exp[k, :3].tolist())
+                                      for k in bad[:5]])
+    if not O.striped_available():
+        return
+    qs = [om.map(b.query(k)) for k in range(n)]
+    ts = [om.map(b.target(k)) for k in range(n)]
+    for name, alg, fl in (("sw", O.SW, (0, 0, 0, 0)), ("nw", O.NW, (0, 0, 0, 0)),
+                          ("sg", O.SG, (1, 1, 1, 1))):
+        exp, _, secs = O.striped_batch(alg, qs, ts, 11, 1, om, fl, threads=16)
+        assert secs >= 0
+        r = res[name]
+        got = np.stack([r["score"], r["end_query"], r["end_ref"]], axis=1)
+        bad = np.nonzero((got != exp).any(axis=1))[0]
+        assert bad.size == 0, (name, [(int(k), got[k].tolist(), exp[k].tolist())
                                       for k in bad[:5]])
 
 
