@@ -196,7 +196,11 @@ static void resident_free(phip_resident_t *r);
 static int resident_enqueue4(phip_resident_t *r, int npass, const int *alg, const int *sg_flags,
                              int open, int gap);
 
-static phip_resident_t *resident_create(const phip_batch_t *b)
+// wait: return once the uploads have ended.  A one-call batch passes false:
+// its host sources live until the call returns, every later use of the
+// uploads is ordered after them on the stream, and the host plans the pass
+// meanwhile; resident_free waits for the stream before it frees.
+static phip_resident_t *resident_create(const phip_batch_t *b, bool wait = true)
 {
     phip_resident_t *r = new phip_resident_t();
     const int np = b->npairs;
@@ -258,7 +262,7 @@ static phip_resident_t *resident_create(const phip_batch_t *b)
     ok &= hipMemcpyAsync(r->dtlen, b->tlen, np * 4, hipMemcpyHostToDevice, g_stream) == hipSuccess;
     if (!r->troff.empty())
         ok &= hipMemcpyAsync(r->dtroff, r->troff.data(), np * 8, hipMemcpyHostToDevice, g_stream) == hipSuccess;
-    ok &= hipStreamSynchronize(g_stream) == hipSuccess;
+    if (wait || !ok) ok &= hipStreamSynchronize(g_stream) == hipSuccess;
     if (!ok) {
         phip_set_error("resident batch upload failed");
         dev_free(r->base);
@@ -298,6 +302,8 @@ static int resident_host_buffers(phip_resident_t *r)
 static void resident_free(phip_resident_t *r)
 {
     if (!r) return;
+    // copies from r's host vectors, or kernels on its blocks, may be in flight
+    (void)hipStreamSynchronize(g_stream);
     if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
     if (r->graph) (void)hipGraphDestroy(r->graph);
     for (int k = 0; k < phip_resident_t::kPasses; ++k) {
@@ -561,7 +567,7 @@ extern "C" int phip_align(const phip_batch_t *b, phip_out_t *out, int8_t **trace
     if (b->npairs == 0) return 0;
     pthread_mutex_lock(&g_lock);
     int rc = -1;
-    phip_resident_t *r = resident_create(b);
+    phip_resident_t *r = resident_create(b, false);
     if (r) {
         rc = resident_run(r, b->alg, b->kind, b->sg_flags, b->open, b->gap, b->want_sat_check);
         if (rc == 0) rc = resident_fetch(r, out, traces);

@@ -154,14 +154,17 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
     int32_t *tl = (int32_t *)malloc(sizeof(int32_t) * (size_t)npairs);
     int64_t *qo = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
     int64_t *to = (int64_t *)malloc(sizeof(int64_t) * (size_t)npairs);
-    phip_out_t *res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)npairs);
-    if (!ql || !tl || !qo || !to || !res) goto oom;
+    /* the results: in the pinned staging buffer with the codes when it can
+     * be had (the download then runs at DMA speed), else ordinary memory */
+    phip_out_t *res = NULL;
+    int res_owned = 0;
+    if (!ql || !tl || !qo || !to) goto oom;
     for (int k = 0; k < npairs; ++k) {
         ql[k] = pssm ? matrix->length : s1_len[k];
         tl[k] = s2_len[k];
         if (ql[k] <= 0 || tl[k] <= 0) {
             fprintf(stderr, "%s: pair %d has an empty sequence\n", __func__, k);
-            free(ql); free(tl); free(qo); free(to); free(res);
+            free(ql); free(tl); free(qo); free(to);
             return -1;
         }
         qo[k] = qtot;
@@ -177,7 +180,9 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
          * staging buffer when it can be had, else to ordinary memory */
         const size_t sq = ((size_t)(s1 ? qtot + 1 : 0) + 255) & ~(size_t)255;
         const size_t st = ((size_t)ttot + 1 + 255) & ~(size_t)255;
-        uint8_t *stage = (uint8_t *)phip_stage_acquire((sq + st) * (dev_cigar ? 2 : 1));
+        const size_t scodes = (sq + st) * (dev_cigar ? 2 : 1);
+        uint8_t *stage =
+            (uint8_t *)phip_stage_acquire(scodes + sizeof(phip_out_t) * (size_t)npairs);
         uint8_t *q = NULL, *t = NULL, *qraw = NULL, *traw = NULL;
         if (stage) {
             q = s1 ? stage : NULL;
@@ -186,6 +191,7 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
                 qraw = stage + sq + st;
                 traw = qraw + sq;
             }
+            res = (phip_out_t *)(stage + scodes);
         } else {
             q = s1 ? (uint8_t *)malloc((size_t)qtot + 1) : NULL;
             t = (uint8_t *)malloc((size_t)ttot + 1);
@@ -193,12 +199,15 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
                 qraw = (uint8_t *)malloc((size_t)qtot + 1);
                 traw = (uint8_t *)malloc((size_t)ttot + 1);
             }
+            res = (phip_out_t *)malloc(sizeof(phip_out_t) * (size_t)npairs);
+            res_owned = 1;
         }
         phip_cigars_t cg;
         memset(&cg, 0, sizeof(cg));
-        if ((s1 && !q) || !t || (dev_cigar && (!qraw || !traw))) {
+        if ((s1 && !q) || !t || (dev_cigar && (!qraw || !traw)) || !res) {
             if (stage) phip_stage_release(stage);
-            else { free(q); free(t); free(qraw); free(traw); }
+            else { free(q); free(t); free(qraw); free(traw); free(res); }
+            res = NULL;
             goto oom;
         }
         if (dev_cigar) {
@@ -211,7 +220,8 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
             if (!cg.ops || !cg.off || !cg.nops || !cg.beg) {
                 free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
                 if (stage) phip_stage_release(stage);
-                else { free(q); free(t); free(qraw); free(traw); }
+                else { free(q); free(t); free(qraw); free(traw); free(res); }
+                res = NULL;
                 goto oom;
             }
         }
@@ -248,13 +258,14 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
         b.toff = to;
         b.tlen = tl;
         int rc = dev_cigar ? phip_align_cigars(&b, qraw, traw, res, &cg) : phip_align(&b, res, NULL);
-        if (stage) phip_stage_release(stage);
-        else { free(q); free(t); free(qraw); free(traw); }
+        if (!stage) { free(q); free(t); free(qraw); free(traw); }
         if (rc != 0) {
             snprintf(g_err, sizeof(g_err), "%s", phip_last_error());
             fprintf(stderr, "%s: %s\n", __func__, g_err);
             free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
-            free(ql); free(tl); free(qo); free(to); free(res);
+            if (stage) phip_stage_release(stage);
+            if (res_owned) free(res);
+            free(ql); free(tl); free(qo); free(to);
             return -1;
         }
         {
@@ -262,12 +273,15 @@ PHOST_EXPORT int parasail_hip_align_batch(const char *alg_name, int kind, int wi
             /* the CIGAR objects are two mallocs each: several threads share them */
             phost_parallel_for(npairs, dev_cigar ? 25000 : 50000, convert_range, &cc);
         }
+        /* the results lived in the staging buffer until converted */
+        if (stage) phip_stage_release(stage);
+        if (res_owned) free(res);
         free(cg.ops); free(cg.off); free(cg.nops); free(cg.beg);
     }
-    free(ql); free(tl); free(qo); free(to); free(res);
+    free(ql); free(tl); free(qo); free(to);
     return 0;
 oom:
-    free(ql); free(tl); free(qo); free(to); free(res);
+    free(ql); free(tl); free(qo); free(to);
     fprintf(stderr, "%s: out of host memory\n", __func__);
     return -1;
 }
