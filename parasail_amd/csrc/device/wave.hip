@@ -73,7 +73,8 @@ struct WaveJob {
     int64_t cdoff[2]; // lane candidates (int4 units, T per pair)
 };
 
-// One queue entry: up to 64/T jobs that share R, T and the loop bounds.
+// One queue entry: up to 64/T jobs that share R, T, the band count and the
+// loop bounds.
 struct WaveDesc {
     int job0, njobs, R, T;
     int nb, width, bbw, prio;
@@ -1002,7 +1003,11 @@ WavePlan *wave_plan_create(const HostBatch &hb, const DPParams &dp, hipStream_t 
             w.R = R;
             double cst = 0;
             size_t j1 = j0;
-            while (j1 < jobs.size() && (int)(j1 - j0) < 64 / T && jT[j1] == T && jR[j1] == R) {
+            // one band count per wave: the wave runs its most bands over its
+            // widest columns, so a many-band job beside a wide one would
+            // multiply the two (one such wave once took twice a pass)
+            while (j1 < jobs.size() && (int)(j1 - j0) < 64 / T && jT[j1] == T && jR[j1] == R &&
+                   jobs[j1].nb == jobs[j0].nb) {
                 w.nb = std::max(w.nb, jobs[j1].nb);
                 w.width = std::max(w.width, std::max(jobs[j1].n[0], jobs[j1].n[1]));
                 cst = std::max(cst, jcost[j1]);
