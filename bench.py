@@ -147,6 +147,35 @@ def per_call_gcups(reps=1, threads=1):
     return b.cells / best / 1e9, b.cells, best
 
 
+def per_call_driver(threads=(1, 16)):
+    """Config 0 through parasail_sw_striped_profile_hip_64_16 from C threads,
+    as parasail's OpenMP driver calls it (tests/test_openmp.c:353-369):
+    the query and the 2,000 targets written as FASTA, timed by the built
+    parasail_amd/percall_omp.  Returns {threads: (GCUPS, seconds)}."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "parasail_amd", "percall_omp")
+    if not os.path.exists(exe):
+        raise RuntimeError("per-call driver not built: run __graft_entry__.build()")
+    q, b = WL.config0()
+    with tempfile.TemporaryDirectory() as d:
+        fq, fd = os.path.join(d, "q.fa"), os.path.join(d, "db.fa")
+        with open(fq, "wb") as f:
+            f.write(b">q\n" + q + b"\n")
+        with open(fd, "wb") as f:
+            for k in range(len(b)):
+                f.write(b">t%d\n" % k + b.target(k) + b"\n")
+        out = subprocess.run([exe, "sw_striped_profile_hip_64_16", fq, fd, "11", "1"] +
+                             [str(t) for t in threads], check=True, capture_output=True,
+                             text=True, timeout=300).stdout
+    res = {}
+    for line in out.split("\n"):
+        f = line.split()
+        if len(f) == 4:
+            res[int(f[0])] = (float(f[2]), float(f[1]))
+    return res
+
+
 def per_call_cpu(threads):
     """Config 0 on the host: the 300-residue query against the 2,000 targets
     through the restated AVX2 striped SW at 16 bits (the
@@ -634,7 +663,13 @@ def main():
                                    "gcups_16_threads": round(g16, 3),
                                    "seconds_16_threads": round(s16, 3),
                                    "function": "parasail_sw_striped_profile_hip_64_16",
-                                   "workload": "1 x 300 aa query vs 2,000 targets U[50,1000]"}
+                                   "workload": "1 x 300 aa query vs 2,000 targets U[50,1000]",
+                                   "threads_from": "Python threads (ctypes)"}
+        drv = per_call_driver((1, 16, 32))
+        line["per_pair_kernel"]["c_openmp"] = {
+            "gcups_by_threads": {str(t): round(v[0], 3) for t, v in sorted(drv.items())},
+            "driver": "parasail_amd/percall_omp: OpenMP over the targets, one profile, each "
+                      "result freed (tests/test_openmp.c:353-369), best of 3"}
         if "sw" in per_alg:
             line["batched_over_per_pair_sw"] = round(per_alg["sw"] / g, 1)
         if not a.no_cpu:

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libparasail_hip.so")
+DRIVER = os.path.join(HERE, "percall_omp")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PARASAIL_HIP_ARCH", "gfx950")
 
@@ -60,6 +61,11 @@ def build(verbose=False, jobs=None):
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
          ["-lpthread", "-lz"])
     os.replace(tmp, LIB)
+    # bench.py's per-call driver (OpenMP over targets, as parasail's own tests
+    # call the per-pair functions); not part of the library
+    _run(["gcc", "-O2", "-fopenmp", "-Wall", "-o", DRIVER,
+          os.path.join(CSRC, "bench", "percall_omp.c"), "-L" + HERE, "-lparasail_hip",
+          "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 
