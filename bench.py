@@ -641,6 +641,9 @@ def main():
                                "BLOSUM62 11/1, SW+NW+SG score-only, int16-first sat",
                    "pairs": len(batch), "cells_per_alg": cells},
         "gcups_kernel_by_alg": per_alg,
+        # each pass's main kernel (HIP events) at its fixed ops per cell
+        "frac_by_alg": {alg: round(cells * OPS[alg] / (np.mean(mms[alg]) / 1e3) / peak_ops, 4)
+                        for alg in algs},
         "graph": {"captures": gcap.value, "replays": grep_.value,
                   "ms_per_step_without_graph": round(elapsed_ng / a.steps * 1e3, 2)},
         "autotune": dict(tune, tuned_ms_per_step=round(ms_per_step, 2)) if tune else None,
